@@ -44,13 +44,17 @@ typedef enum {
   PHOBIC_E_NOT_FOUND = 10        /* serving: unknown request id */
 } phobic_status;
 
-/* MPHFconfig (include/phobicGpu/mphf_config.h:16-65) plus two extensions
- * whose defaults reproduce the reference exactly. */
+/* MPHFconfig (include/phobicGpu/mphf_config.h:16-65).  With buckets = 0 and
+ * fulcrums = NULL this is the default constructor (mphf_config.h:26-32):
+ * B = ceil(P / lambda) and the OptBucketer fulcrum table.  A custom Bucketer
+ * (mphf_config.h:34-42) passes its B and its 2048-entry table.  `seed` is an
+ * extension; 0 reproduces the reference's XXH128(ptr, len, 0). */
 typedef struct {
-  double lambda;           /* averageBucketSize; B = ceil(P / lambda) */
-  uint32_t partition_size; /* P; partitions hold at most P + P/2 keys */
-  uint32_t flags;          /* reserved, must be 0 */
-  uint64_t seed;           /* XXH3 seed; the reference uses 0 (hasher.hpp:34) */
+  double lambda;              /* averageBucketSize */
+  uint32_t partition_size;    /* P; partitions hold at most P + P/2 keys */
+  uint32_t buckets;           /* 0 = ceil(P / lambda); else B for `fulcrums` */
+  uint64_t seed;              /* XXH3 seed */
+  const uint32_t* fulcrums;   /* NULL = OptBucketer; else 2048 entries */
 } phobic_config;
 
 /* The reference's pre-hashed Key (include/phobicGpu/hasher.hpp:7-20). */
@@ -84,6 +88,10 @@ int phobic_query_sizes(const phobic_config* cfg, uint64_t n, uint32_t* np, uint3
 
 /* The 2048-entry fulcrum table (mphf_config.h:44-55, bucketers.h:28-39). */
 int phobic_fulcrums(const phobic_config* cfg, uint32_t* fulcs_out);
+
+/* Host XXH3-128 of a byte string (the query-side initial hash,
+ * hasher.hpp:33-61); lo64/hi64 receive the low and high halves. */
+int phobic_xxh3_128(const void* data, uint64_t len, uint64_t seed, uint64_t* lo64, uint64_t* hi64);
 
 /* ---- builds from host buffers (synchronous, like MPHFbuilder::build) ---- */
 

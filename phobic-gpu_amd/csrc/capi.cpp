@@ -47,8 +47,9 @@ struct DevBuf {
   }
 };
 
-constexpr int kStages = 3;
-const char* const kStageNames[kStages] = {"hash_partition_scatter", "partition_offsets", "partition_search"};
+constexpr int kStages = 4;
+const char* const kStageNames[kStages] = {"hash_route", "partition_split", "partition_offsets", "partition_search"};
+constexpr uint32_t kMaxLevels = 4;
 
 }  // namespace
 
@@ -56,14 +57,12 @@ struct phobic_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   // scratch owned by the context
-  DevBuf pcount, rec_lower, rec_bkt, fulcs, status;
+  DevBuf pcount, lvl_count, rec_lo[2], rec_meta[2], fulcs, status;
   // staging for host-buffer entry points
   DevBuf in_a, in_b, out_pilots, out_offsets;
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
-  // fulcrum cache
-  double f_lambda = -1;
-  uint32_t f_P = 0;
+  // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
 
@@ -86,19 +85,53 @@ int hip_status(hipError_t e) { return e == hipSuccess ? PHOBIC_OK : (e == hipErr
 struct Plan {
   uint64_t n;
   uint32_t np, B, cap;
+  // partitioning levels: level l (1..L) bins cover G[l] partitions, G[L] = 1
+  uint32_t L, fanout;
+  uint64_t G[kMaxLevels + 1];
 };
+
+// Fan-out f <= kMaxFanout and the fewest levels L with f^L >= np.
+void plan_levels(Plan* pl) {
+  const uint32_t np = pl->np;
+  if (np <= kMaxFanout) {
+    pl->L = 1;
+    pl->fanout = np ? np : 1;
+  } else {
+    uint32_t L = 2;
+    while (std::pow((double)kMaxFanout, (double)L) < (double)np) L++;
+    uint32_t f = (uint32_t)std::ceil(std::pow((double)np, 1.0 / L));
+    auto reach = [&](uint32_t ff) {
+      uint64_t r = 1;
+      for (uint32_t i = 0; i < L; i++) r *= ff;
+      return r;
+    };
+    while (f > 2 && reach(f - 1) >= np) f--;
+    while (reach(f) < np) f++;
+    pl->L = L;
+    pl->fanout = f;
+  }
+  for (uint32_t l = 1; l <= pl->L; l++) {
+    uint64_t g = 1;
+    for (uint32_t i = l; i < pl->L; i++) g *= pl->fanout;
+    pl->G[l] = g;
+  }
+}
 
 int validate(const phobic_config* cfg, uint64_t n, Plan* pl) {
   if (!cfg || !(cfg->lambda > 0.0) || !std::isfinite(cfg->lambda)) return PHOBIC_E_INVALID_CONFIG;
-  if (cfg->partition_size < 2 || cfg->partition_size > (1u << 20) || cfg->flags != 0) return PHOBIC_E_INVALID_CONFIG;
+  if (cfg->partition_size < 2 || cfg->partition_size > (1u << 20)) return PHOBIC_E_INVALID_CONFIG;
   if (n > 0xFFFFFFFFull) return PHOBIC_E_INVALID_CONFIG;  // offsets are u32 (mphf_builder.h:254)
-  const double Bd = std::ceil((double)cfg->partition_size / cfg->lambda);
+  if ((cfg->buckets != 0) != (cfg->fulcrums != nullptr)) return PHOBIC_E_INVALID_CONFIG;
+  const double Bd = cfg->buckets ? (double)cfg->buckets : std::ceil((double)cfg->partition_size / cfg->lambda);
   if (!(Bd >= 1.0) || Bd > 65535.0) return PHOBIC_E_INVALID_CONFIG;
   pl->n = n;
   pl->B = (uint32_t)Bd;
   pl->np = (uint32_t)((n + cfg->partition_size - 1) / cfg->partition_size);
   pl->cap = cfg->partition_size + cfg->partition_size / 2;  // mphf_config.h:61-64
   if (search_lds_bytes(pl->cap, pl->B) > 64 * 1024) return PHOBIC_E_INVALID_CONFIG;
+  if (pl->np > (1u << 21)) return PHOBIC_E_INVALID_CONFIG;  // partition index inside a bin must fit 16 bits
+  plan_levels(pl);
+  if (pl->L > kMaxLevels || pl->G[1] > 65535) return PHOBIC_E_INVALID_CONFIG;
   return PHOBIC_OK;
 }
 
@@ -117,6 +150,27 @@ void compute_fulcrums(double lambda, uint32_t P, uint32_t* f) {
   f[kFulcsTable - 1] = B << 16;
 }
 
+// The table a build uses: the caller's (custom Bucketer) or OptBucketer's.
+void config_fulcrums(const phobic_config* cfg, uint32_t* out) {
+  if (cfg->fulcrums) {
+    std::memcpy(out, cfg->fulcrums, kFulcsTable * 4);
+  } else {
+    compute_fulcrums(cfg->lambda, cfg->partition_size, out);
+  }
+}
+
+// Uploads the configuration's fulcrums unless the context already holds them.
+int upload_fulcrums(phobic_ctx* ctx, const phobic_config* cfg, hipStream_t s) {
+  uint32_t f[kFulcsTable];
+  config_fulcrums(cfg, f);
+  if (ctx->fulcs.ptr && std::memcmp(f, ctx->f_host, sizeof(f)) == 0) return PHOBIC_OK;
+  std::memcpy(ctx->f_host, f, sizeof(f));
+  TRY_HIP(ctx->fulcs.ensure(kFulcsTable * 4));
+  TRY_HIP(hipMemcpyAsync(ctx->fulcs.ptr, ctx->f_host, kFulcsTable * 4, hipMemcpyHostToDevice, s));
+  TRY_HIP(hipStreamSynchronize(s));
+  return PHOBIC_OK;
+}
+
 int set_device(phobic_ctx* ctx) {
   TRY_HIP(hipSetDevice(ctx->device));
   return PHOBIC_OK;
@@ -125,49 +179,82 @@ int set_device(phobic_ctx* ctx) {
 // Enqueue one build.  Inputs/outputs are device pointers.
 int enqueue_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_config* cfg, const Plan& pl,
                   uint32_t* d_pilots, uint32_t* d_offsets, uint32_t* d_status, hipStream_t s) {
-  if (ctx->f_lambda != cfg->lambda || ctx->f_P != cfg->partition_size) {
-    compute_fulcrums(cfg->lambda, cfg->partition_size, ctx->f_host);
-    ctx->f_lambda = cfg->lambda;
-    ctx->f_P = cfg->partition_size;
-    TRY_HIP(ctx->fulcs.ensure(kFulcsTable * 4));
-    TRY_HIP(hipMemcpy(ctx->fulcs.ptr, ctx->f_host, kFulcsTable * 4, hipMemcpyHostToDevice));
+  TRY(upload_fulcrums(ctx, cfg, s));
+  const uint32_t np = pl.np, L = pl.L, f = pl.fanout;
+  const uint32_t bins1 = (uint32_t)((np + pl.G[1] - 1) / pl.G[1]);
+  const uint64_t slots = (uint64_t)bins1 * pl.G[1] * pl.cap;
+  // per-level bin counters: level l has ceil(np / G[l-1]) * f bins (level 1: bins1)
+  uint64_t cnt_off[kMaxLevels + 1] = {}, cnt_total = 0;
+  for (uint32_t l = 1; l < L; l++) {
+    cnt_off[l] = cnt_total;
+    cnt_total += l == 1 ? bins1 : ((np + pl.G[l - 1] - 1) / pl.G[l - 1]) * (uint64_t)f;
   }
-  const uint64_t slots = (uint64_t)pl.np * pl.cap;
-  TRY_HIP(ctx->pcount.ensure((size_t)pl.np * 4));
-  TRY_HIP(ctx->rec_lower.ensure(slots * 8));
-  TRY_HIP(ctx->rec_bkt.ensure(slots * 2));
+  TRY_HIP(ctx->pcount.ensure(((uint64_t)np + f) * 4));
+  TRY_HIP(ctx->lvl_count.ensure((cnt_total + 1) * 4));
+  for (int i = 0; i < 2; i++) {
+    TRY_HIP(ctx->rec_lo[i].ensure(slots * 8));
+    TRY_HIP(ctx->rec_meta[i].ensure(slots * 4));
+  }
   TRY_HIP(launch_fill_u32(d_status, 1, 0, s));
-  TRY_HIP(launch_fill_u32(ctx->pcount.as<uint32_t>(), pl.np, 0, s));
+  TRY_HIP(launch_fill_u32(ctx->pcount.as<uint32_t>(), (uint64_t)np + f, 0, s));
+  TRY_HIP(launch_fill_u32(ctx->lvl_count.as<uint32_t>(), cnt_total, 0, s));
   TRY_HIP(hipEventRecord(ctx->ev[0], s));
 
-  ScatterArgs sa{};
-  sa.src = src;
-  sa.n = pl.n;
-  sa.seed = cfg->seed;
-  sa.np = pl.np;
-  sa.cap = pl.cap;
-  sa.fulcs = ctx->fulcs.as<uint32_t>();
-  sa.pcount = ctx->pcount.as<uint32_t>();
-  sa.rec_lower = ctx->rec_lower.as<uint64_t>();
-  sa.rec_bkt = ctx->rec_bkt.as<uint16_t>();
-  sa.status = d_status;
-  TRY_HIP(launch_hash_scatter(kind, sa, s));
+  // level l writes buffer 0 when L - l is even, so level L lands in buffer 0
+  auto out_buf = [&](uint32_t l) { return (int)((L - l) & 1); };
+  auto counts = [&](uint32_t l) {
+    return l == L ? ctx->pcount.as<uint32_t>() : ctx->lvl_count.as<uint32_t>() + cnt_off[l];
+  };
+  HashRouteArgs ha{};
+  ha.src = src;
+  ha.n = pl.n;
+  ha.seed = cfg->seed;
+  ha.np = np;
+  ha.cap = pl.cap;
+  ha.G = (uint32_t)pl.G[1];
+  ha.fanout = bins1;
+  ha.fulcs = ctx->fulcs.as<uint32_t>();
+  ha.child_count = counts(1);
+  ha.out_lo = ctx->rec_lo[out_buf(1)].as<uint64_t>();
+  ha.out_meta = ctx->rec_meta[out_buf(1)].as<uint32_t>();
+  ha.status = d_status;
+  TRY_HIP(launch_hash_route(kind, ha, s));
   TRY_HIP(hipEventRecord(ctx->ev[1], s));
 
-  TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), pl.np, d_offsets, s));
+  for (uint32_t l = 2; l <= L; l++) {
+    SplitArgs sp{};
+    const uint32_t parents = (uint32_t)((np + pl.G[l - 1] - 1) / pl.G[l - 1]);
+    sp.in_lo = ctx->rec_lo[out_buf(l - 1)].as<uint64_t>();
+    sp.in_meta = ctx->rec_meta[out_buf(l - 1)].as<uint32_t>();
+    sp.parent_count = counts(l - 1);
+    sp.parent_cap = pl.G[l - 1] * pl.cap;
+    sp.slices = (uint32_t)((sp.parent_cap + route_tile_records() - 1) / route_tile_records());
+    sp.fanout = f;
+    sp.G_child = (uint32_t)pl.G[l];
+    sp.cap = pl.cap;
+    sp.child_count = counts(l);
+    sp.out_lo = ctx->rec_lo[out_buf(l)].as<uint64_t>();
+    sp.out_meta = ctx->rec_meta[out_buf(l)].as<uint32_t>();
+    sp.status = d_status;
+    TRY_HIP(launch_split_level(sp, parents, s));
+  }
   TRY_HIP(hipEventRecord(ctx->ev[2], s));
 
+  TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, d_offsets, s));
+  TRY_HIP(hipEventRecord(ctx->ev[3], s));
+
   SearchArgs ra{};
-  ra.rec_lower = ctx->rec_lower.as<uint64_t>();
-  ra.rec_bkt = ctx->rec_bkt.as<uint16_t>();
+  ra.rec_lower = ctx->rec_lo[0].as<uint64_t>();
+  ra.rec_meta = ctx->rec_meta[0].as<uint32_t>();
   ra.pcount = ctx->pcount.as<uint32_t>();
-  ra.np = pl.np;
+  ra.sorted = ctx->rec_lo[1].as<uint64_t>();  // free once partitioning is done
+  ra.np = np;
   ra.B = pl.B;
   ra.cap = pl.cap;
   ra.pilots = d_pilots;
   ra.status = d_status;
   TRY_HIP(launch_partition_search(ra, s));
-  TRY_HIP(hipEventRecord(ctx->ev[3], s));
+  TRY_HIP(hipEventRecord(ctx->ev[4], s));
   ctx->timed = true;
   return PHOBIC_OK;
 }
@@ -268,7 +355,7 @@ int phobic_ctx_destroy(phobic_ctx* ctx) {
   if (!ctx) return PHOBIC_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (DevBuf* b : {&ctx->pcount, &ctx->rec_lower, &ctx->rec_bkt, &ctx->fulcs, &ctx->status, &ctx->in_a, &ctx->in_b,
+  for (DevBuf* b : {&ctx->pcount, &ctx->lvl_count, &ctx->rec_lo[0], &ctx->rec_lo[1], &ctx->rec_meta[0], &ctx->rec_meta[1], &ctx->fulcs, &ctx->status, &ctx->in_a, &ctx->in_b,
                     &ctx->out_pilots, &ctx->out_offsets})
     b->release();
   for (int i = 0; i <= kStages; i++)
@@ -307,7 +394,7 @@ int phobic_fulcrums(const phobic_config* cfg, uint32_t* out) {
   Plan pl;
   TRY(validate(cfg, 0, &pl));
   if (!out) return PHOBIC_E_INVALID_CONFIG;
-  compute_fulcrums(cfg->lambda, cfg->partition_size, out);
+  config_fulcrums(cfg, out);
   return PHOBIC_OK;
 }
 
@@ -374,13 +461,7 @@ int phobic_eval_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t cou
   Plan pl;
   TRY(validate(cfg, 0, &pl));
   TRY(set_device(ctx));
-  if (ctx->f_lambda != cfg->lambda || ctx->f_P != cfg->partition_size) {
-    compute_fulcrums(cfg->lambda, cfg->partition_size, ctx->f_host);
-    ctx->f_lambda = cfg->lambda;
-    ctx->f_P = cfg->partition_size;
-    TRY_HIP(ctx->fulcs.ensure(kFulcsTable * 4));
-    TRY_HIP(hipMemcpy(ctx->fulcs.ptr, ctx->f_host, kFulcsTable * 4, hipMemcpyHostToDevice));
-  }
+  TRY(upload_fulcrums(ctx, cfg, (hipStream_t)stream));
   EvalArgs ea{};
   ea.keys = d_keys;
   ea.count = count;

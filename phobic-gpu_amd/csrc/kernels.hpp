@@ -25,22 +25,44 @@ struct KeySource {
   const uint4* keys;      // kKindKey: pre-hashed 16-byte keys
 };
 
-struct ScatterArgs {
+constexpr uint32_t kMaxFanout = 128;  // child bins per partitioning level
+
+// Level 1 of the partitioning: hash keys and route them to level-1 bins.
+struct HashRouteArgs {
   KeySource src;
   uint64_t n;
   uint64_t seed;
   uint32_t np, cap;
+  uint32_t G;             // partitions per level-1 bin
+  uint32_t fanout;        // number of level-1 bins, <= kMaxFanout
   const uint32_t* fulcs;  // 2048 entries
-  uint32_t* pcount;       // np, zeroed before the launch
-  uint64_t* rec_lower;    // np*cap: lower1 | lower2 << 32
-  uint16_t* rec_bkt;      // np*cap: bucket inside the partition
+  uint32_t* child_count;  // per level-1 bin, zeroed before the launch
+  uint64_t* out_lo;       // records: lower1 | lower2 << 32
+  uint32_t* out_meta;     // records: partition-in-bin << 16 | bucket
   uint32_t* status;       // error bit mask
+};
+
+// Levels 2..L: split every parent bin into `fanout` child bins.
+struct SplitArgs {
+  const uint64_t* in_lo;
+  const uint32_t* in_meta;
+  const uint32_t* parent_count;
+  uint64_t parent_cap;    // G_parent * cap
+  uint32_t slices;        // tiles per parent bin (grid = parents * slices)
+  uint32_t fanout;        // children per parent
+  uint32_t G_child;       // partitions per child bin
+  uint32_t cap;
+  uint32_t* child_count;  // per child bin, zeroed before the launch
+  uint64_t* out_lo;
+  uint32_t* out_meta;
+  uint32_t* status;
 };
 
 struct SearchArgs {
   const uint64_t* rec_lower;
-  const uint16_t* rec_bkt;
+  const uint32_t* rec_meta;  // bucket in the low 16 bits
   const uint32_t* pcount;
+  uint64_t* sorted;  // np*cap scratch: keys of each partition in bucket order
   uint32_t np, B, cap;
   uint32_t* pilots;  // np*B, interleaved p + b*np
   uint32_t* status;
@@ -58,7 +80,9 @@ struct EvalArgs {
 };
 
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
-hipError_t launch_hash_scatter(int kind, const ScatterArgs& a, hipStream_t s);
+hipError_t launch_hash_route(int kind, const HashRouteArgs& a, hipStream_t s);
+hipError_t launch_split_level(const SplitArgs& a, uint32_t parents, hipStream_t s);
+uint32_t route_tile_records();
 hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_t* offsets, hipStream_t s);
 size_t search_lds_bytes(uint32_t cap, uint32_t B);
 hipError_t launch_partition_search(const SearchArgs& a, hipStream_t s);

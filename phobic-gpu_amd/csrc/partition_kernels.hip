@@ -1,0 +1,212 @@
+// partition_kernels.hip -- hashing and multi-level radix partitioning of keys
+// into fixed-capacity partition regions (gfx950).
+//
+// The reference counts bucket sizes and scatters keys with one global atomic
+// per key (bucket_sizes.comp:21-35, redistribute_keys.comp:21-53).  Random
+// per-key global atomics sustain only a few tens of G/s on MI355X, so here
+// keys are routed to their partition in L levels of fan-out f <= 128: every
+// level loads a tile of 4096 records, sorts it by child bin in LDS, takes one
+// global atomic per (tile, child bin) and writes coalesced runs.
+//
+// Level l's bins cover G_l = f^(L-l) consecutive partitions and hold at most
+// G_l * (P + P/2) records, so a full bin implies a partition with more than
+// P + P/2 keys (PHOBIC_E_PARTITION_OVERFLOW, the reference's
+// partitionMaxSize(), mphf_config.h:61-64).  The last level (G = 1) leaves
+// every partition p at records [p*cap, p*cap + count[p]).
+//
+// Record = lower64 (lower1 | lower2 << 32) + meta32 (partition index inside
+// the current bin << 16 | bucket index inside the partition).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_hash.hpp"
+#include "kernels.hpp"
+
+namespace phobic {
+namespace {
+
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kPer = 16;  // records per thread per tile
+constexpr uint32_t kTile = kThreads * kPer;
+
+struct TileLds {
+  uint64_t lo[kTile];
+  uint32_t meta[kTile];
+  uint8_t child[kTile];
+  uint32_t hist[kMaxFanout];
+  uint32_t lstart[kMaxFanout];
+  uint32_t gbase[kMaxFanout];
+  uint32_t wsum[kThreads / 64];
+};
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Routes the block's tile (record k of thread t valid when valid[k]) to
+// child bins child_first + child[k] of capacity child_cap.
+__device__ __forceinline__ void route_tile(TileLds& s, const uint64_t (&lo)[kPer], const uint32_t (&meta)[kPer],
+                                           const uint32_t (&child)[kPer], const bool (&valid)[kPer],
+                                           uint32_t fanout, uint64_t child_first, uint64_t child_cap,
+                                           uint32_t* __restrict__ child_count, uint64_t* __restrict__ out_lo,
+                                           uint32_t* __restrict__ out_meta, uint32_t* __restrict__ status) {
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < fanout; i += kThreads) s.hist[i] = 0;
+  __syncthreads();
+  uint32_t rank[kPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; k++)
+    if (valid[k]) rank[k] = atomicAdd(&s.hist[child[k]], 1u);
+  __syncthreads();
+  // block exclusive scan of the histogram (fanout <= kThreads)
+  const uint32_t v = tid < fanout ? s.hist[tid] : 0;
+  const uint32_t incl = wave_incl_scan(v);
+  if ((tid & 63) == 63) s.wsum[tid >> 6] = incl;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kThreads / 64; w++) {
+    const uint32_t x = s.wsum[w];
+    if (w < (tid >> 6)) before += x;
+    total += x;
+  }
+  if (tid < fanout) {
+    s.lstart[tid] = before + incl - v;
+    s.gbase[tid] = v ? atomicAdd(&child_count[child_first + tid], v) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; k++) {
+    if (valid[k]) {
+      const uint32_t d = s.lstart[child[k]] + rank[k];
+      s.lo[d] = lo[k];
+      s.meta[d] = meta[k];
+      s.child[d] = (uint8_t)child[k];
+    }
+  }
+  __syncthreads();
+  bool over = false;
+  for (uint32_t d = tid; d < total; d += kThreads) {
+    const uint32_t c = s.child[d];
+    const uint64_t pos = (uint64_t)s.gbase[c] + (d - s.lstart[c]);
+    if (pos < child_cap) {
+      const uint64_t at = (child_first + c) * child_cap + pos;
+      out_lo[at] = s.lo[d];
+      out_meta[at] = s.meta[d];
+    } else {
+      over = true;
+    }
+  }
+  if (over) atomicOr(status, 1u << kErrPartitionOverflow);
+  __syncthreads();
+}
+
+template <int KIND>
+__device__ __forceinline__ U128 hash_key(const KeySource& src, uint64_t i, uint64_t seed) {
+  if constexpr (KIND == kKindU64) {
+    return xxh3_128_u64(src.u64[i], seed);
+  } else if constexpr (KIND == kKindBytes) {
+    const uint64_t a = src.offs[i], b = src.offs[i + 1];
+    return xxh3_128_bytes(src.bytes + a, b - a, seed);
+  } else {
+    const uint4 k = src.keys[i];
+    return U128{(uint64_t)k.x | ((uint64_t)k.y << 32), (uint64_t)k.z | ((uint64_t)k.w << 32)};
+  }
+}
+
+}  // namespace
+
+// Level 1: hash raw keys, map to (partition, bucket) and route to the
+// level-1 bins (partitions [c*G, (c+1)*G)).
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
+  __shared__ TileLds s;
+  __shared__ uint32_t fl[kFulcs];
+  for (uint32_t i = threadIdx.x; i < kFulcs; i += kThreads) fl[i] = a.fulcs[i];
+  __syncthreads();
+  const uint64_t ntiles = (a.n + kTile - 1) / kTile;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    uint64_t lo[kPer];
+    uint32_t meta[kPer], child[kPer];
+    bool valid[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      const uint64_t i = t * kTile + k * kThreads + threadIdx.x;
+      valid[k] = i < a.n;
+      lo[k] = 0;
+      meta[k] = 0;
+      child[k] = 0;
+      if (valid[k]) {
+        const U128 h = hash_key<KIND>(a.src, i, a.seed);
+        const uint32_t p = partition_of((uint32_t)h.lo, a.np);
+        const uint32_t b = bucket_of((uint32_t)(h.lo >> 32), fl);
+        const uint32_t c = p / a.G;
+        lo[k] = h.hi;
+        meta[k] = ((p - c * a.G) << 16) | b;
+        child[k] = c;
+      }
+    }
+    route_tile(s, lo, meta, child, valid, a.fanout, 0, (uint64_t)a.G * a.cap, a.child_count, a.out_lo, a.out_meta,
+               a.status);
+  }
+}
+
+// Level l > 1: split every parent bin (G_parent partitions) into `fanout`
+// child bins of G_child partitions.  Block (b, s) handles tile s of parent b.
+__global__ __launch_bounds__(kThreads) void k_split_level(SplitArgs a) {
+  __shared__ TileLds s;
+  const uint32_t parent = blockIdx.x / a.slices;
+  const uint64_t tile0 = (uint64_t)(blockIdx.x % a.slices) * kTile;
+  const uint32_t cnt = min(a.parent_count[parent], (uint32_t)min((uint64_t)0xFFFFFFFFu, a.parent_cap));
+  if (tile0 >= cnt) return;
+  const uint64_t in0 = (uint64_t)parent * a.parent_cap;
+  uint64_t lo[kPer];
+  uint32_t meta[kPer], child[kPer];
+  bool valid[kPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; k++) {
+    const uint64_t i = tile0 + k * kThreads + threadIdx.x;
+    valid[k] = i < cnt;
+    lo[k] = 0;
+    meta[k] = 0;
+    child[k] = 0;
+    if (valid[k]) {
+      lo[k] = a.in_lo[in0 + i];
+      const uint32_t m = a.in_meta[in0 + i];
+      const uint32_t pl = m >> 16;
+      const uint32_t c = pl / a.G_child;
+      meta[k] = ((pl - c * a.G_child) << 16) | (m & 0xFFFFu);
+      child[k] = c;
+    }
+  }
+  route_tile(s, lo, meta, child, valid, a.fanout, (uint64_t)parent * a.fanout, (uint64_t)a.G_child * a.cap,
+             a.child_count, a.out_lo, a.out_meta, a.status);
+}
+
+uint32_t route_tile_records() { return kTile; }
+
+hipError_t launch_hash_route(int kind, const HashRouteArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  const uint64_t ntiles = (a.n + kTile - 1) / kTile;
+  const uint32_t grid = (uint32_t)(ntiles < 2048 ? ntiles : 2048);
+  switch (kind) {
+    case kKindU64: hipLaunchKernelGGL(k_hash_route<kKindU64>, dim3(grid), dim3(kThreads), 0, s, a); break;
+    case kKindBytes: hipLaunchKernelGGL(k_hash_route<kKindBytes>, dim3(grid), dim3(kThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_hash_route<kKindKey>, dim3(grid), dim3(kThreads), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_split_level(const SplitArgs& a, uint32_t parents, hipStream_t s) {
+  if (parents == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_split_level, dim3(parents * a.slices), dim3(kThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace phobic

@@ -1,0 +1,379 @@
+// search_kernel.hip -- per-partition bucket sort + pilot search (gfx950).
+//
+// One wave64 per partition; a workgroup holds kWavesPerGroup independent
+// partitions and every wave synchronises only with itself.  Per partition:
+//   1. bucket sizes from the partition's records (LDS atomics)
+//      -- bucket_sizes.comp:21-35
+//   2. processing order: size descending, bucket index descending
+//      -- sort_sum.comp:156-199 (serial lane-0 loop in the reference; here
+//      a ballot-ranked counting sort)
+//   3. bucket start offsets inside the partition -- sort_sum.comp:203-225
+//   4. keys written to the partition's slice of `sorted` in bucket order
+//      -- redistribute_keys.comp:21-53
+//   5. pilot search -- search.comp:31-175: for every bucket in order, the
+//      smallest V = d + ps*pilot such that the positions
+//      (hash(l1, hash(l2, pilot)) >> 1) mod ps + d (mod ps) are pairwise
+//      distinct and free; they are then marked taken.
+//
+// Slot state lives in an extended bitmap E of L = 2*ps + 128 bits with
+// E[i] = taken[i mod ps], so the 64 candidate offsets [D, D+64) of a key at
+// initial position p are one unaligned 64-bit read of E at p + D.  A trial
+// first tests offsets 0..63 with one window per key and a DPP OR-reduction;
+// if none is free it runs one bit-parallel pass in which lane l owns
+// offsets [64 l, 64 l + 64) and ORs the windows of all keys.  Both return
+// the smallest free offset, which is the reference's answer.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_hash.hpp"
+#include "kernels.hpp"
+
+namespace phobic {
+namespace {
+
+constexpr uint32_t kWavesPerGroup = 4;
+constexpr uint32_t kMaxBucket = 256;  // sort_sum.comp BINS, search.comp MAX_BUCKET_SIZE
+
+struct Lds {
+  uint32_t E_words, coll_words, B;
+  __device__ static uint32_t words(uint32_t cap, uint32_t B) {
+    return ((2 * cap + 128) / 32 + 4) + ((cap + 31) / 32 + 1) + 5 * B + 256;
+  }
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t l = __lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Orders this wave's LDS / global accesses before and after the call for
+// all lanes of the wave (the partition is owned by one wave).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <int CTRL, int ROWS, bool BOUND>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, BOUND);
+}
+
+// OR over the 64 lanes (all lanes active), uniform result.
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+  v |= dpp<0x111, 0xF, true>(v);   // row_shr:1
+  v |= dpp<0x112, 0xF, true>(v);   // row_shr:2
+  v |= dpp<0x114, 0xF, true>(v);   // row_shr:4
+  v |= dpp<0x118, 0xF, true>(v);   // row_shr:8
+  v |= dpp<0x142, 0xA, false>(v);  // row_bcast:15
+  v |= dpp<0x143, 0xC, false>(v);  // row_bcast:31
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// 64 bits of E starting at bit x.
+__device__ __forceinline__ uint64_t window64(const uint32_t* E, uint32_t x) {
+  const uint32_t w = x >> 5, sh = x & 31;
+  const uint32_t a = E[w], b = E[w + 1], c = E[w + 2];
+  return ((uint64_t)__builtin_amdgcn_alignbit(c, b, sh) << 32) | __builtin_amdgcn_alignbit(b, a, sh);
+}
+
+// Lemire's fastmod (M = floor((2^64-1)/d) + 1): a mod d for 32-bit a, d.
+__device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t M, uint32_t d) {
+  return (uint32_t)__umul64hi(M * a, (uint64_t)d);
+}
+
+// readlane of r[k >> 6] from lane k & 63 (k wave-uniform)
+__device__ __forceinline__ uint32_t pick4(const uint32_t (&r)[4], uint32_t k) {
+  const uint32_t c = k >> 6;
+  const uint32_t src = c == 0 ? r[0] : c == 1 ? r[1] : c == 2 ? r[2] : r[3];
+  return __builtin_amdgcn_readlane(src, k & 63);
+}
+
+__device__ __forceinline__ uint64_t low_mask(uint32_t bits) {  // bits >= 64 -> all ones
+  return bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(SearchArgs a) {
+  extern __shared__ __align__(16) uint32_t smem[];
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t cap = a.cap, B = a.B;
+  const uint32_t EW = (2 * cap + 128) / 32 + 4, CW = (cap + 31) / 32 + 1;
+  uint32_t* E = smem + wave * Lds::words(cap, B);
+  uint32_t* coll = E + EW;
+  uint32_t* bcnt = coll + CW;   // key count per bucket
+  uint32_t* bcur = bcnt + B;    // placement cursor
+  uint32_t* bstart = bcur + B;  // start inside the partition
+  uint32_t* order = bstart + B; // bucket index by processing rank
+  uint32_t* pil = order + B;    // pilot per bucket index
+  uint32_t* hist = pil + B;     // 256 bins, t = 256 - size
+
+  const uint32_t p = blockIdx.x * kWavesPerGroup + wave;
+  if (p >= a.np) return;
+  const uint32_t ps = a.pcount[p];
+  const uint64_t base = (uint64_t)p * cap;
+  if (ps > cap) {  // the scatter stage already raised kErrPartitionOverflow
+    for (uint32_t b = lane; b < B; b += 64) a.pilots[p + (uint64_t)b * a.np] = 0;
+    return;
+  }
+  for (uint32_t i = lane; i < EW; i += 64) E[i] = 0;
+  for (uint32_t i = lane; i < CW; i += 64) coll[i] = 0;
+  for (uint32_t i = lane; i < B; i += 64) {
+    bcnt[i] = 0;
+    bcur[i] = 0;
+    pil[i] = 0;
+  }
+  for (uint32_t i = lane; i < 256; i += 64) hist[i] = 0;
+  wave_sync();
+
+  // 1. bucket sizes
+  for (uint32_t i = lane; i < ps; i += 64) atomicAdd(&bcnt[a.rec_meta[base + i] & 0xFFFFu], 1u);
+  wave_sync();
+  bool over = false;
+  for (uint32_t b = lane; b < B; b += 64) {
+    const uint32_t s = bcnt[b];
+    if (s > kMaxBucket) over = true;
+    else if (s) atomicAdd(&hist[256 - s], 1u);
+  }
+  if (__any(over)) {
+    if (lane == 0) atomicOr(a.status, 1u << kErrBucketOverflow);
+    for (uint32_t b = lane; b < B; b += 64) a.pilots[p + (uint64_t)b * a.np] = 0;
+    return;
+  }
+  wave_sync();
+
+  // 2a. bin starts: exclusive scan over t = 256 - size (largest first)
+  uint32_t m;
+  {
+    uint32_t v[4], loc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      v[j] = hist[4 * lane + j];
+      loc += v[j];
+    }
+    const uint32_t incl = wave_incl_scan(loc);
+    uint32_t run = incl - loc;
+    m = __builtin_amdgcn_readlane(incl, 63);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      hist[4 * lane + j] = run;
+      run += v[j];
+    }
+  }
+  wave_sync();
+
+  // 2b. ranks: within one size, larger bucket index first
+  for (uint32_t c = 0; c < B; c += 64) {
+    const int idx = (int)B - 1 - (int)(c + lane);
+    const uint32_t s = idx >= 0 ? bcnt[idx] : 0;
+    bool act = s > 0;
+    uint64_t am = __ballot(act);
+    while (am) {
+      const uint32_t leader = (uint32_t)__builtin_ctzll(am);
+      const uint32_t v = __builtin_amdgcn_readlane(s, leader);
+      const bool mine = act && s == v;
+      const uint64_t same = __ballot(mine);
+      const uint32_t t = 256 - v;
+      const uint32_t bv = hist[t];
+      if (mine) {
+        order[bv + __popcll(same & lanemask_lt())] = (uint32_t)idx;
+        act = false;
+      }
+      wave_sync();
+      if (lane == leader) hist[t] = bv + __popcll(same);
+      wave_sync();
+      am &= ~same;
+    }
+  }
+  wave_sync();
+
+  // 3. bucket start offsets
+  {
+    uint32_t carry = 0;
+    for (uint32_t r0 = 0; r0 < m; r0 += 64) {
+      const uint32_t r = r0 + lane;
+      const uint32_t b = r < m ? order[r] : 0;
+      const uint32_t s = r < m ? bcnt[b] : 0;
+      const uint32_t incl = wave_incl_scan(s);
+      if (r < m) bstart[b] = carry + incl - s;
+      carry += __builtin_amdgcn_readlane(incl, 63);
+    }
+  }
+  wave_sync();
+
+  // 4. keys in processing order into this partition's slice of `sorted`
+  uint64_t* __restrict__ sorted = a.sorted + base;
+  for (uint32_t i = lane; i < ps; i += 64) {
+    const uint32_t b = a.rec_meta[base + i] & 0xFFFFu;
+    const uint32_t pos = bstart[b] + atomicAdd(&bcur[b], 1u);
+    sorted[pos] = a.rec_lower[base + i];
+  }
+  wave_sync();
+
+  // 5. pilot search
+  const uint64_t M = ~0ull / (ps ? ps : 1) + 1;
+  const uint32_t L = 2 * ps + 128;  // valid bits of E
+  const uint32_t nblk = (ps + 63) / 64;
+  const uint64_t blk_mask = lane < nblk ? low_mask(ps - 64 * lane) : 0ull;
+  uint32_t err = 0;
+
+  uint64_t cur[4];
+  uint32_t cs = 0, cst = 0, cb = 0;
+  if (m) {
+    cb = order[0];
+    cs = bcnt[cb];
+    cst = bstart[cb];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t k = lane + 64 * j;
+      cur[j] = k < cs ? sorted[cst + k] : 0;
+    }
+  }
+  for (uint32_t r = 0; r < m; r++) {
+    // prefetch the next bucket's keys
+    uint64_t nxt[4];
+    uint32_t ns = 0, nst = 0, nb = 0;
+    if (r + 1 < m) {
+      nb = order[r + 1];
+      ns = bcnt[nb];
+      nst = bstart[nb];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t k = lane + 64 * j;
+      nxt[j] = k < ns ? sorted[nst + k] : 0;
+    }
+
+    const uint32_t s = cs;
+    uint32_t pos[4] = {0, 0, 0, 0};
+    uint32_t pilot = 0, dsel = 0;
+    for (;;) {
+      const uint64_t vbase = (uint64_t)ps * pilot;
+      if (vbase > 0x7FFFFFFFull) {
+        err = kErrPilotOverflow;
+        break;
+      }
+      const uint32_t dlimit = (uint32_t)min((uint64_t)ps, 0x80000000ull - vbase);  // d < dlimit
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t k = lane + 64 * j;
+        if (k < s) pos[j] = fastmod(mix32((uint32_t)cur[j], mix32((uint32_t)(cur[j] >> 32), pilot)) >> 1, M, ps);
+      }
+      if (s > 1) {
+        bool c = false;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint32_t k = lane + 64 * j;
+          if (k < s) {
+            const uint32_t msk = 1u << (pos[j] & 31);
+            c |= (atomicOr(&coll[pos[j] >> 5], msk) & msk) != 0;
+          }
+        }
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint32_t k = lane + 64 * j;
+          if (k < s) coll[pos[j] >> 5] = 0;
+        }
+        wave_sync();
+        if (__any(c)) {
+          if (pilot == 0) {  // identical keys collide for every pilot
+            bool dup = false;
+            for (uint32_t k2 = 0; k2 < s; k2++) {
+              const uint64_t kv = sorted[cst + k2];
+#pragma unroll
+              for (int j = 0; j < 4; j++) {
+                const uint32_t k = lane + 64 * j;
+                if (k < s && k != k2 && cur[j] == kv) dup = true;
+              }
+            }
+            if (__any(dup)) {
+              err = kErrDuplicateKeys;
+              break;
+            }
+          }
+          pilot++;
+          continue;
+        }
+      }
+      // offsets 0..63: one window per key, OR over keys
+      uint64_t t0 = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t k = lane + 64 * j;
+        if (k < s) t0 |= window64(E, pos[j]);
+      }
+      const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
+      const uint64_t free0 = ~taken0 & low_mask(dlimit);
+      if (free0) {
+        dsel = (uint32_t)__builtin_ctzll(free0);
+        break;
+      }
+      if (dlimit > 64) {
+        // all offsets: lane l owns [64 l, 64 l + 64)
+        uint64_t acc = 0;
+        const uint32_t off = 64 * (lane < nblk ? lane : 0);
+        for (uint32_t k = 0; k < s; k++) acc |= window64(E, pick4(pos, k) + off);
+        const uint64_t lim = lane < nblk ? low_mask(dlimit > 64 * lane ? dlimit - 64 * lane : 0) : 0ull;
+        const uint64_t fr = ~acc & blk_mask & lim;
+        const uint64_t any = __ballot(fr != 0);
+        if (any) {
+          const uint32_t l0 = (uint32_t)__builtin_ctzll(any);
+          const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)fr, l0);
+          const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(fr >> 32), l0);
+          const uint64_t f = ((uint64_t)hi << 32) | lo;
+          dsel = 64 * l0 + (uint32_t)__builtin_ctzll(f);
+          break;
+        }
+      }
+      pilot++;
+    }
+    if (err) break;
+    // mark the bucket's slots in every copy of E
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t k = lane + 64 * j;
+      if (k < s) {
+        uint32_t q = pos[j] + dsel;
+        if (q >= ps) q -= ps;
+        for (uint32_t x = q; x < L; x += ps) atomicOr(&E[x >> 5], 1u << (x & 31));
+      }
+    }
+    if (lane == 0) pil[cb] = dsel + ps * pilot;
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < 4; j++) cur[j] = nxt[j];
+    cs = ns;
+    cst = nst;
+    cb = nb;
+  }
+  if (err && lane == 0) atomicOr(a.status, 1u << err);
+  wave_sync();
+  for (uint32_t b = lane; b < B; b += 64) a.pilots[p + (uint64_t)b * a.np] = pil[b];
+}
+
+size_t search_lds_bytes(uint32_t cap, uint32_t B) {
+  const size_t words = ((2 * (size_t)cap + 128) / 32 + 4) + ((cap + 31) / 32 + 1) + 5 * (size_t)B + 256;
+  return words * 4 * kWavesPerGroup;
+}
+
+hipError_t launch_partition_search(const SearchArgs& a, hipStream_t s) {
+  if (a.np == 0) return hipSuccess;
+  const uint32_t groups = (a.np + kWavesPerGroup - 1) / kWavesPerGroup;
+  hipLaunchKernelGGL(k_partition_search, dim3(groups), dim3(64 * kWavesPerGroup), search_lds_bytes(a.cap, a.B), s, a);
+  return hipGetLastError();
+}
+
+}  // namespace phobic

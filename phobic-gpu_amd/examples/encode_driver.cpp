@@ -1,0 +1,66 @@
+// encode_driver.cpp -- runs the host half of the build (MPHF::setData with the
+// FastQueryMphf encoders, serialisation and queries) on pilots / offsets read
+// from a file.  Needs no GPU.
+//   encode_driver IN OUT [QUERY_KEYS QUERY_OUT]
+// IN:  u64 n, u64 np, u64 B, u64 P, f64 lambda, u64 seed,
+//      u32 pilots[np*B] (interleaved), u32 offsets[np+1]
+// OUT: the serialised MPHF bytes.  QUERY_KEYS: u64 count + u64 keys;
+// QUERY_OUT: u32 positions.
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include <phobic_gpu_mphf.hpp>
+
+using namespace phobicgpu;
+
+template <typename T>
+static void rd(std::ifstream& in, T* p, size_t count) {
+  in.read(reinterpret_cast<char*>(p), sizeof(T) * count);
+  if (!in) throw std::runtime_error("short read");
+}
+
+int main(int argc, char** argv) {
+  if (argc != 3 && argc != 5) {
+    std::cerr << "usage: encode_driver IN OUT [QUERY_KEYS QUERY_OUT]" << std::endl;
+    return 2;
+  }
+  std::ifstream in(argv[1], std::ios::binary);
+  uint64_t n, np, B, P, seed;
+  double lambda;
+  rd(in, &n, 1);
+  rd(in, &np, 1);
+  rd(in, &B, 1);
+  rd(in, &P, 1);
+  rd(in, &lambda, 1);
+  rd(in, &seed, 1);
+  std::vector<uint32_t> pilots(np * B), offsets(np + 1);
+  rd(in, pilots.data(), pilots.size());
+  rd(in, offsets.data(), offsets.size());
+  MPHFconfig cfg(lambda, (uint32_t)P);
+  cfg.seed = seed;
+  if (cfg.bucketCountPerPartition != B) {
+    std::cerr << "bucket count mismatch" << std::endl;
+    return 1;
+  }
+  FastQueryMphf f;
+  f.setData(pilots, offsets, (uint32_t)np, cfg);
+  const std::vector<uint8_t> bytes = f.serialize();
+  std::ofstream out(argv[2], std::ios::binary);
+  out.write(reinterpret_cast<const char*>(bytes.data()), bytes.size());
+  if (argc == 5) {
+    std::ifstream qin(argv[3], std::ios::binary);
+    uint64_t count;
+    rd(qin, &count, 1);
+    std::vector<uint64_t> keys(count);
+    rd(qin, keys.data(), count);
+    std::vector<uint32_t> pos(count);
+    for (uint64_t i = 0; i < count; i++) pos[i] = f(keys[i]);
+    std::ofstream qout(argv[4], std::ios::binary);
+    qout.write(reinterpret_cast<const char*>(pos.data()), pos.size() * 4);
+  }
+  std::printf("bits_per_key=%.4f bytes=%zu\n", n ? f.getBitsPerKey() : 0.0f, bytes.size());
+  return 0;
+}

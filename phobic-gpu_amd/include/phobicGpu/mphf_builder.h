@@ -1,0 +1,107 @@
+// mphf_builder.h -- MPHFbuilder::build (reference
+// include/phobicGpu/mphf_builder.h:18-321).  The whole construction runs on the
+// MI355X through the C ABI of phobic_hip.h (hash, partitioning, bucket sort,
+// pilot search); the host stages the keys, reads back pilots and offsets and
+// hands them to MPHF::setData, as BuildInvocation::run does.
+#pragma once
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "host_timer.h"
+#include "mphf.hpp"
+#include "mphf_config.h"
+#include "phobic_hip.h"
+
+namespace phobicgpu {
+
+class MPHFbuilder {
+ private:
+  MPHFconfig config;
+  phobic_ctx* ctx = nullptr;
+  int device;
+
+  static void check(int st, const char* what) {
+    if (st != PHOBIC_OK) throw std::runtime_error(std::string(what) + ": " + phobic_status_string(st));
+  }
+
+  phobic_config abiConfig() const {
+    phobic_config c{};
+    c.lambda = config.m_averageBucketSize;
+    c.partition_size = config.partitionSize;
+    c.seed = config.seed;
+    if (config.hasCustomBucketer()) {
+      c.buckets = config.bucketCountPerPartition;
+      c.fulcrums = config.getFulcs().data();
+    }
+    return c;
+  }
+
+ public:
+  explicit MPHFbuilder(MPHFconfig config = MPHFconfig(), int device = 0) : config(config), device(device) {}
+  MPHFbuilder(const MPHFbuilder&) = delete;
+  MPHFbuilder& operator=(const MPHFbuilder&) = delete;
+  ~MPHFbuilder() {
+    if (ctx) phobic_ctx_destroy(ctx);
+  }
+
+  const MPHFconfig& getConfig() const { return config; }
+
+  template <typename Mphf, typename keyType>
+  HostTimer build(const std::vector<keyType>& keys, Mphf& f) {
+    HostTimer timer;
+    if (!ctx) check(phobic_ctx_create(device, &ctx), "phobic_ctx_create");
+    timer.addLabel("allocation");
+    const phobic_config cfg = abiConfig();
+    const uint64_t n = keys.size();
+    uint32_t np = 0, B = 0;
+    check(phobic_query_sizes(&cfg, n, &np, &B), "phobic_query_sizes");
+    std::vector<uint32_t> pilots((size_t)np * B), offsets((size_t)np + 1);
+    phobic_stage_times t{};
+    int st;
+    if constexpr (Mphf::noHash()) {
+      static_assert(std::is_same_v<keyType, Key>, "nohash MPHFs are built from Key values");
+      st = phobic_build_keys(ctx, reinterpret_cast<const phobic_key*>(keys.data()), n, &cfg, pilots.data(),
+                             offsets.data(), &t);
+    } else if constexpr (std::is_same_v<keyType, uint64_t>) {
+      st = phobic_build_u64(ctx, keys.data(), n, &cfg, pilots.data(), offsets.data(), &t);
+    } else if constexpr (std::is_same_v<keyType, std::string>) {
+      std::vector<uint64_t> offs(n + 1, 0);
+      for (uint64_t i = 0; i < n; i++) offs[i + 1] = offs[i] + keys[i].size();
+      std::vector<uint8_t> bytes(offs[n]);
+      for (uint64_t i = 0; i < n; i++)
+        if (!keys[i].empty()) std::memcpy(bytes.data() + offs[i], keys[i].data(), keys[i].size());
+      timer.addLabel("input_packing");
+      st = phobic_build_bytes(ctx, bytes.data(), offs.data(), n, &cfg, pilots.data(), offsets.data(), &t);
+    } else if constexpr (std::is_same_v<keyType, std::pair<uint64_t, uint64_t>>) {
+      std::vector<uint64_t> offs(n + 1), words(2 * n);
+      for (uint64_t i = 0; i < n; i++) {
+        offs[i] = 16 * i;
+        words[2 * i] = keys[i].first;
+        words[2 * i + 1] = keys[i].second;
+      }
+      offs[n] = 16 * n;
+      st = phobic_build_bytes(ctx, reinterpret_cast<const uint8_t*>(words.data()), offs.data(), n, &cfg,
+                              pilots.data(), offsets.data(), &t);
+    } else {
+      static_assert(sizeof(keyType) == 0, "supported key types: uint64_t, std::string, std::pair<u64,u64>, Key");
+    }
+    check(st, "MPHFbuilder::build");
+    const double gpu0 = timer.elapsed() - t.total_ms * 1e6;
+    double acc = gpu0;
+    for (uint32_t i = 0; i < t.count; i++) {
+      acc += t.ms[i] * 1e6;
+      timer.addLabelManually(std::string("GPU_") + t.names[i], acc);
+    }
+    timer.addLabel("result_transfer");
+    f.setData(pilots, offsets, np, config);
+    timer.addLabel("encoding");
+    return timer;
+  }
+};
+
+}  // namespace phobicgpu

@@ -1,9 +1,9 @@
 """ctypes binding of libphobic_hip.so (the C ABI in include/phobic_hip.h).
 
 Thin plumbing for tests and benchmarks: numpy arrays for the host-buffer
-entry points, torch tensors (device pointers + the current HIP stream) for
-the device entry points.  Loading fails loudly when the HIP library has not
-been built; there is no CPU fallback.
+entry points, hip_rt.DeviceBuffer objects (or raw device addresses) and
+hip_rt.Stream handles for the device entry points.  Loading fails loudly
+when the HIP library has not been built; there is no CPU fallback.
 """
 from __future__ import annotations
 
@@ -43,8 +43,9 @@ class Config(ctypes.Structure):
     _fields_ = [
         ("lambda_", ctypes.c_double),
         ("partition_size", ctypes.c_uint32),
-        ("flags", ctypes.c_uint32),
+        ("buckets", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
+        ("fulcrums", ctypes.c_void_p),
     ]
 
 
@@ -69,6 +70,7 @@ EXPORTS = [
     "phobic_status_string",
     "phobic_query_sizes",
     "phobic_fulcrums",
+    "phobic_xxh3_128",
     "phobic_build_u64",
     "phobic_build_bytes",
     "phobic_build_keys",
@@ -101,6 +103,7 @@ def load() -> ctypes.CDLL:
         "phobic_status_string": (ctypes.c_char_p, [I]),
         "phobic_query_sizes": (I, [cfgp, U64, ctypes.POINTER(U32), ctypes.POINTER(U32)]),
         "phobic_fulcrums": (I, [cfgp, P]),
+        "phobic_xxh3_128": (I, [P, U64, U64, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
         "phobic_build_u64": (I, [P, P, U64, cfgp, P, P, tp]),
         "phobic_build_bytes": (I, [P, P, P, U64, cfgp, P, P, tp]),
         "phobic_build_keys": (I, [P, P, U64, cfgp, P, P, tp]),
@@ -122,7 +125,7 @@ def load() -> ctypes.CDLL:
 
 def make_config(lam: float = 8.0, partition_size: int = 2048, seed: int = 0) -> Config:
     """MPHFconfig(averageBucketSize, partitionSize) defaults (mphf_config.h:26)."""
-    return Config(float(lam), int(partition_size), 0, int(seed))
+    return Config(float(lam), int(partition_size), 0, int(seed), None)
 
 
 def sizes(cfg: Config, n: int) -> tuple[int, int]:
@@ -144,19 +147,18 @@ def fulcrums(cfg: Config) -> np.ndarray:
 
 
 def _ptr(a) -> int:
+    """numpy array (host), hip_rt.DeviceBuffer, or a raw integer address."""
     if a is None:
         return 0
     if isinstance(a, np.ndarray):
         return a.ctypes.data
-    return a.data_ptr()  # torch tensor
+    if isinstance(a, int):
+        return a
+    return int(a.ptr)
 
 
 def _stream(stream=None) -> int:
-    if stream is not None:
-        return int(stream)
-    import torch
-
-    return int(torch.cuda.current_stream().cuda_stream)
+    return int(stream) if stream is not None else 0
 
 
 @dataclass
@@ -222,10 +224,10 @@ class Context:
         buf = data if data.size else np.zeros(1, dtype=np.uint8)
         return self._host(self.lib.phobic_build_bytes, (buf.ctypes.data, offsets.ctypes.data, n), cfg, n)
 
-    # ---- device builds (torch tensors; enqueue on the current stream) ----
-    def build_u64_device(self, d_keys, cfg: Config, d_pilots, d_offsets, d_status, stream=None) -> None:
+    # ---- device builds (device buffers; enqueue on `stream`, default null) ----
+    def build_u64_device(self, d_keys, n, cfg: Config, d_pilots, d_offsets, d_status, stream=None) -> None:
         st = self.lib.phobic_build_u64_device(
-            self.h, _ptr(d_keys), d_keys.numel(), ctypes.byref(cfg), _ptr(d_pilots), _ptr(d_offsets),
+            self.h, _ptr(d_keys), n, ctypes.byref(cfg), _ptr(d_pilots), _ptr(d_offsets),
             _ptr(d_status), _stream(stream))
         if st:
             raise PhobicError(st, "build_u64_device")
@@ -251,23 +253,23 @@ class Context:
             raise PhobicError(st, "last_stage_times")
         return t.as_dict()
 
-    def eval_u64_device(self, d_keys, cfg, np_, d_pilots, d_offsets, d_out, stream=None) -> None:
+    def eval_u64_device(self, d_keys, count, cfg, np_, d_pilots, d_offsets, d_out, stream=None) -> None:
         st = self.lib.phobic_eval_u64_device(
-            self.h, _ptr(d_keys), d_keys.numel(), ctypes.byref(cfg), np_, _ptr(d_pilots), _ptr(d_offsets),
+            self.h, _ptr(d_keys), count, ctypes.byref(cfg), np_, _ptr(d_pilots), _ptr(d_offsets),
             _ptr(d_out), _stream(stream))
         if st:
             raise PhobicError(st, "eval_u64_device")
 
-    def check_bijection_device(self, d_pos, n, d_bitmap, d_bad, stream=None) -> None:
+    def check_bijection_device(self, d_pos, count, n, d_bitmap, d_bad, stream=None) -> None:
         st = self.lib.phobic_check_bijection_device(
-            self.h, _ptr(d_pos), d_pos.numel(), n, _ptr(d_bitmap), _ptr(d_bad), _stream(stream))
+            self.h, _ptr(d_pos), count, n, _ptr(d_bitmap), _ptr(d_bad), _stream(stream))
         if st:
             raise PhobicError(st, "check_bijection_device")
 
 
-def gen_splitmix64_device(seed: int, d_keys, stream=None) -> None:
+def gen_splitmix64_device(seed: int, n: int, d_keys, stream=None) -> None:
     lib = load()
-    st = lib.phobic_gen_splitmix64_device(seed, d_keys.numel(), _ptr(d_keys), _stream(stream))
+    st = lib.phobic_gen_splitmix64_device(seed, n, _ptr(d_keys), _stream(stream))
     if st:
         raise PhobicError(st, "gen_splitmix64_device")
 

@@ -1,0 +1,208 @@
+"""Parity of the HIP construction path with the oracle (bit-exact pilots and
+partition offsets), through the C ABI, on an MI355X."""
+import os
+
+import numpy as np
+import pytest
+
+import hip_rt
+import phobic_hip as ph
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def oracle_u64(keys, lam, P, seed):
+    hk = po.hash_u64(keys, seed)
+    st, pil, off = po.build_keys(hk, lam, P)
+    return st, pil, off, hk
+
+
+def assert_bijection(hk, pil, off, lam, P):
+    q = po.query_keys(hk, pil, off, lam, P)
+    n = hk.shape[0]
+    if n:
+        assert int(q.max()) < n and np.unique(q).size == n
+
+
+def test_c1_bit_exact_vs_golden(ctx):
+    g = np.load(os.path.join(GOLD, "c1_pilots.npz"))
+    keys = po.splitmix64(1, 1_000_000)
+    r = ctx.build_u64(keys, ph.make_config(7.0, 2048, 0))
+    assert np.array_equal(r.offsets, g["offsets"])
+    assert np.array_equal(r.pilots, g["pilots"])
+
+
+@pytest.mark.parametrize("n,lam,P,seed,kseed", [
+    (1_000_000, 7.0, 2048, 42, 1),   # 1M-key configuration with hash seed 42
+    (2_000_000, 8.0, 2500, 0, 2),    # 100M-key configuration's lambda / P
+    (1_000_000, 7.5, 2048, 0, 3),    # 1B-key configuration's lambda / P
+    (1_000_000, 6.0, 2048, 0, 4),    # string configuration's lambda / P
+    (500_000, 4.5, 2500, 0, 5),      # reference example.cpp configuration
+    (200_000, 12.0, 1024, 7, 6),
+    (5_000, 3.0, 64, 0, 7),
+])
+def test_u64_bit_exact_vs_oracle(ctx, n, lam, P, seed, kseed):
+    keys = po.splitmix64(kseed, n)
+    r = ctx.build_u64(keys, ph.make_config(lam, P, seed))
+    st, pil, off, hk = oracle_u64(keys, lam, P, seed)
+    assert st == 0
+    assert np.array_equal(r.offsets, off)
+    assert np.array_equal(r.pilots, pil)
+    assert_bijection(hk, r.pilots, r.offsets, lam, P)
+
+
+def random_strings(n, lo, hi, seed, printable=True):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lo, hi + 1, n)
+    offs = np.zeros(n + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    if printable:
+        data = rng.integers(0x20, 0x7F, int(offs[-1]), dtype=np.uint8)
+    else:
+        data = rng.integers(0, 256, int(offs[-1]), dtype=np.uint8)
+    return data, offs
+
+
+def test_strings_bit_exact(ctx):
+    data, offs = random_strings(300_000, 8, 64, 3)
+    r = ctx.build_bytes(data, offs, ph.make_config(6.0, 2048, 0))
+    hk = po.hash_bytes(data, offs, 0)
+    st, pil, off = po.build_keys(hk, 6.0, 2048)
+    assert st == 0
+    assert np.array_equal(r.offsets, off) and np.array_equal(r.pilots, pil)
+
+
+@pytest.mark.parametrize("seed", [0, 42, 2**64 - 1])
+def test_strings_every_xxh3_length_class(ctx, seed):
+    # lengths 0..300 cover the 0, 1-3, 4-8, 9-16, 17-128, 129-240 and long paths
+    rng = np.random.default_rng(seed % 1000)
+    lens = np.concatenate([np.arange(0, 301), [511, 1024, 1025, 4096, 5000]])
+    lens = np.repeat(lens, 20)
+    offs = np.zeros(lens.size + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    data = rng.integers(0, 256, int(offs[-1]), dtype=np.uint8)
+    # distinct strings: the first 8 bytes of each carry its index (len >= 8);
+    # short strings are deduplicated below
+    for i in range(lens.size):
+        if lens[i] >= 8:
+            data[int(offs[i]):int(offs[i]) + 8] = np.frombuffer(np.uint64(i).tobytes(), np.uint8)
+    strs = {}
+    for i in range(lens.size):
+        strs.setdefault(data[int(offs[i]):int(offs[i + 1])].tobytes(), i)
+    keep = sorted(strs.values())
+    lens2 = lens[keep]
+    offs2 = np.zeros(len(keep) + 1, np.uint64)
+    offs2[1:] = np.cumsum(lens2)
+    data2 = np.concatenate([data[int(offs[i]):int(offs[i + 1])] for i in keep])
+    cfg = ph.make_config(5.0, 256, seed)
+    r = ctx.build_bytes(data2, offs2, cfg)
+    hk = po.hash_bytes(data2, offs2, seed)
+    st, pil, off = po.build_keys(hk, 5.0, 256)
+    assert st == 0
+    assert np.array_equal(r.offsets, off) and np.array_equal(r.pilots, pil)
+
+
+def test_prehashed_keys_bit_exact(ctx):
+    hk = po.hash_u64(po.splitmix64(8, 700_000), 0)
+    r = ctx.build_keys(hk, ph.make_config(7.0, 2048))
+    st, pil, off = po.build_keys(hk, 7.0, 2048)
+    assert np.array_equal(r.offsets, off) and np.array_equal(r.pilots, pil)
+
+
+def test_empty_and_tiny(ctx):
+    r = ctx.build_u64(np.zeros(0, np.uint64), ph.make_config(7.0, 2048))
+    assert r.pilots.size == 0 and list(r.offsets) == [0]
+    for n in (1, 2, 3, 64, 2049):
+        keys = po.splitmix64(100 + n, n)
+        r = ctx.build_u64(keys, ph.make_config(7.0, 2048))
+        st, pil, off, hk = oracle_u64(keys, 7.0, 2048, 0)
+        assert np.array_equal(r.offsets, off) and np.array_equal(r.pilots, pil)
+
+
+def test_duplicate_keys_error(ctx):
+    keys = po.splitmix64(9, 10_000)
+    keys[5000] = keys[17]
+    with pytest.raises(ph.PhobicError) as e:
+        ctx.build_u64(keys, ph.make_config(7.0, 2048))
+    assert e.value.status == 7
+
+
+def test_bucket_overflow_error(ctx):
+    hk = po.hash_u64(po.splitmix64(10, 4000), 0)
+    hk[:300, 0] = 5
+    hk[:300, 1] = 12345
+    with pytest.raises(ph.PhobicError) as e:
+        ctx.build_keys(hk, ph.make_config(7.0, 2048))
+    assert e.value.status == 6
+    assert po.build_keys(hk, 7.0, 2048)[0] == 6
+
+
+def test_partition_overflow_error(ctx):
+    P = 256
+    hk = po.hash_u64(po.splitmix64(12, 4 * P), 0)
+    hk[: P + P // 2 + 1, 0] = 0
+    with pytest.raises(ph.PhobicError) as e:
+        ctx.build_keys(hk, ph.make_config(7.0, P))
+    assert e.value.status == 5
+
+
+def test_context_reuse_after_error(ctx):
+    keys = po.splitmix64(13, 50_000)
+    r = ctx.build_u64(keys, ph.make_config(7.0, 2048))
+    st, pil, off, hk = oracle_u64(keys, 7.0, 2048, 0)
+    assert np.array_equal(r.pilots, pil)
+
+
+def test_deterministic_repeat(ctx):
+    keys = po.splitmix64(14, 2_000_000)
+    cfg = ph.make_config(7.0, 2048)
+    a = ctx.build_u64(keys, cfg)
+    b = ctx.build_u64(keys, cfg)
+    assert np.array_equal(a.pilots, b.pilots) and np.array_equal(a.offsets, b.offsets)
+
+
+def test_device_generator_matches_numpy(ctx):
+    n = 1_000_003
+    d = hip_rt.DeviceBuffer(n * 8)
+    ph.gen_splitmix64_device(77, n, d)
+    hip_rt.synchronize()
+    assert np.array_equal(d.to_numpy(np.uint64), po.splitmix64(77, n))
+
+
+def test_device_build_matches_host_build(ctx):
+    n, lam, P = 3_000_000, 8.0, 2500
+    cfg = ph.make_config(lam, P)
+    np_, B = ph.sizes(cfg, n)
+    dk = hip_rt.DeviceBuffer(n * 8)
+    ph.gen_splitmix64_device(2, n, dk)
+    dp, do, ds = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
+    ctx.build_u64_device(dk, n, cfg, dp, do, ds)
+    hip_rt.synchronize()
+    assert int(ds.to_numpy(np.uint32)[0]) == 0
+    r = ctx.build_u64(po.splitmix64(2, n), cfg)
+    assert np.array_equal(dp.to_numpy(np.uint32), r.pilots)
+    assert np.array_equal(do.to_numpy(np.uint32), r.offsets)
+
+
+def test_device_bijection_100m(ctx):
+    """100M distinct keys (splitmix64 seed 2), lambda 8, P 2500: every key maps
+    to a distinct slot of [0, n), checked on the device."""
+    n, lam, P = 100_000_000, 8.0, 2500
+    cfg = ph.make_config(lam, P)
+    np_, B = ph.sizes(cfg, n)
+    dk = hip_rt.DeviceBuffer(n * 8)
+    ph.gen_splitmix64_device(2, n, dk)
+    dp, do, ds = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
+    ctx.build_u64_device(dk, n, cfg, dp, do, ds)
+    hip_rt.synchronize()
+    assert int(ds.to_numpy(np.uint32)[0]) == 0
+    off = do.to_numpy(np.uint32)
+    assert int(off[-1]) == n and np.all(np.diff(off.astype(np.int64)) <= P + P // 2)
+    out = hip_rt.DeviceBuffer(n * 4)
+    ctx.eval_u64_device(dk, n, cfg, np_, dp, do, out)
+    bm, bad = hip_rt.DeviceBuffer((n + 31) // 32 * 4), hip_rt.DeviceBuffer(8)
+    ctx.check_bijection_device(out, n, n, bm, bad)
+    hip_rt.synchronize()
+    assert int(bad.to_numpy(np.uint64)[0]) == 0

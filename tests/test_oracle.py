@@ -1,0 +1,348 @@
+"""Pins for the CPU oracle (oracle/phobic_oracle.c) -- CPU only.
+
+Each check is independent of the oracle's own code path:
+  * XXH3-128: digests from python xxhash (tests/golden/xxh3_vectors.json)
+  * fulcrums / B: tables printed by the reference's own mphf_config.h
+    (tests/golden/fulcrums.npz, and oracle/_ref/ref_config when built)
+  * hi32 multiply: exact Python integers
+  * mix32: inverted with the inverse of the reference's odd multiplier
+  * bucket mapping: the 64-bit host formulation of mphf.hpp:42-49
+  * bucket order: an independent (-size, -index) sort
+  * search: brute-force minimality over every smaller pilot value
+  * every build: bijection onto [0, n) (example.cpp:43-59)
+"""
+import json
+import os
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _vector_bytes(length, data_seed):
+    words = po.splitmix64(data_seed, (length + 7) // 8)
+    return words.astype("<u8").tobytes()[:length]
+
+
+def test_xxh3_golden_bytes():
+    g = json.load(open(os.path.join(GOLD, "xxh3_vectors.json")))
+    assert len(g["bytes"]) > 1000
+    for v in g["bytes"]:
+        data = _vector_bytes(v["len"], v["data_seed"])
+        assert po.xxh3_128(data, int(v["seed"])) == int(v["digest"], 16), (v["len"], v["seed"])
+
+
+def test_xxh3_golden_u64_key_split():
+    g = json.load(open(os.path.join(GOLD, "xxh3_vectors.json")))
+    for v in g["u64"]:
+        k = np.array([int(v["key"])], dtype=np.uint64)
+        key = po.hash_u64(k, v["seed"])[0]
+        d = int(v["digest"], 16)
+        lo, hi = d & (2**64 - 1), d >> 64
+        # hasher.hpp:36-37: {lo32(low64), hi32(low64), lo32(high64), hi32(high64)}
+        assert [int(x) for x in key] == [lo & 0xFFFFFFFF, lo >> 32, hi & 0xFFFFFFFF, hi >> 32]
+
+
+def test_xxh3_known_answer_survey():
+    # XXH3_128(le64(1), seed 0) = 0xbdc94bce2eda264d_bc08dc21994df8a2
+    key = po.hash_u64(np.array([1], dtype=np.uint64), 0)[0]
+    assert [int(x) for x in key] == [0x994DF8A2, 0xBC08DC21, 0x2EDA264D, 0xBDC94BCE]
+
+
+def test_xxh3_live_xxhash():
+    xxhash = pytest.importorskip("xxhash")
+    rng = random.Random(11)
+    for _ in range(300):
+        ln = rng.choice([rng.randint(0, 300), rng.randint(0, 3000)])
+        seed = rng.getrandbits(64)
+        data = bytes(rng.getrandbits(8) for _ in range(ln))
+        assert po.xxh3_128(data, seed) == xxhash.xxh3_128_intdigest(data, seed)
+
+
+def test_hash_bytes_matches_single():
+    rng = np.random.default_rng(2)
+    lens = rng.integers(0, 100, 500)
+    offs = np.zeros(501, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    data = rng.integers(0, 256, int(offs[-1]), dtype=np.uint8)
+    keys = po.hash_bytes(data, offs, 7)
+    for i in range(0, 500, 37):
+        d = po.xxh3_128(data[int(offs[i]):int(offs[i + 1])].tobytes(), 7)
+        lo, hi = d & (2**64 - 1), d >> 64
+        assert [int(x) for x in keys[i]] == [lo & 0xFFFFFFFF, lo >> 32, hi & 0xFFFFFFFF, hi >> 32]
+
+
+def test_fulcrums_match_reference_tables():
+    g = np.load(os.path.join(GOLD, "fulcrums.npz"))
+    names = [k for k in g.files if k.startswith("f_")]
+    assert len(names) >= 6
+    for k in names:
+        lam, P = k[2:].split("_")
+        lam, P = float(lam), int(P)
+        assert np.array_equal(po.fulcrums(lam, P), g[k]), k
+        B, maxsize = g["b_" + k[2:]]
+        assert po.sizes(1, lam, P)[1] == B
+        assert maxsize == P + P // 2
+
+
+def test_fulcrums_match_reference_binary():
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_config")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/ref_config not built (needs the reference tree)")
+    rng = random.Random(5)
+    for _ in range(25):
+        lam = round(rng.uniform(1.5, 12.0), 3)
+        P = rng.randint(64, 8192)
+        out = subprocess.run([exe, repr(lam), str(P)], capture_output=True, text=True, check=True).stdout.split("\n")
+        B = int(out[0].split()[0])
+        ref = np.array(list(map(int, out[1].split())), dtype=np.uint32)
+        assert np.array_equal(po.fulcrums(lam, P), ref), (lam, P)
+        assert po.sizes(1, lam, P)[1] == B
+
+
+def test_sizes_of_configs():
+    # B = ceil(P / lambda) for the benchmark configurations
+    assert po.sizes(1_000_000, 7.0, 2048) == (489, 293)
+    assert po.sizes(100_000_000, 8.0, 2500) == (40000, 313)
+    assert po.sizes(1_000_000_000, 7.5, 2048) == (488282, 274)
+    assert po.sizes(200_000_000, 6.0, 2048) == (97657, 342)
+
+
+def test_hi32_multiply_exact():
+    lib = po.load()
+    rng = random.Random(3)
+    cases = [(0, 0), (0xFFFFFFFF, 0xFFFFFFFF), (1, 0xFFFFFFFF), (0x80000000, 2)]
+    cases += [(rng.getrandbits(32), rng.getrandbits(32)) for _ in range(20000)]
+    for a, b in cases:
+        assert lib.orc_hi32_mul(a, b) == (a * b) >> 32
+
+
+def _mix_inverse(h, pilot):
+    inv = pow(0x45D9F3B, -1, 2**32)
+
+    def unshift(x):  # x ^ (x >> 16) is its own inverse on 32 bits
+        return x ^ (x >> 16)
+
+    x = unshift(h)
+    x = unshift((x * inv) & 0xFFFFFFFF)
+    x = unshift((x * inv) & 0xFFFFFFFF)
+    return x ^ pilot
+
+
+def test_mix32_inverts_with_reference_multiplier():
+    lib = po.load()
+    rng = random.Random(4)
+    assert lib.orc_mix32(0, 0) == 0
+    for _ in range(20000):
+        k, p = rng.getrandbits(32), rng.getrandbits(32)
+        assert _mix_inverse(lib.orc_mix32(k, p), p) == k
+
+
+def _get_bucket_host(x, f):
+    # mphf.hpp:42-49, 64-bit formulation
+    z = x * 2047
+    idx, part = z >> 32, z & 0xFFFFFFFF
+    v1 = (int(f[idx]) * part) >> 32
+    v2 = (int(f[idx + 1]) * (0xFFFFFFFF - part)) >> 32
+    return (v1 + v2) >> 16
+
+
+def test_bucket_of_matches_host_formulation():
+    lib = po.load()
+    rng = random.Random(6)
+    for lam, P in [(7.0, 2048), (8.0, 2500), (3.0, 64)]:
+        f = po.fulcrums(lam, P)
+        B = po.sizes(1, lam, P)[1]
+        xs = [0, 1, 0xFFFFFFFF, 0x80000000] + [rng.getrandbits(32) for _ in range(20000)]
+        for x in xs:
+            b = lib.orc_bucket_of(x, f.ctypes.data)
+            assert b == _get_bucket_host(x, f)
+            assert b < B
+
+
+def _keys(n, seed=1, hseed=0):
+    return po.hash_u64(po.splitmix64(seed, n), hseed)
+
+
+def _check_bijection(hk, pil, off, lam, P):
+    q = po.query_keys(hk, pil, off, lam, P)
+    n = hk.shape[0]
+    assert q.size == n
+    if n:
+        assert int(q.max()) < n
+        assert np.unique(q).size == n
+
+
+def test_bucket_order_is_size_desc_index_desc():
+    lam, P = 7.0, 2048
+    hk = _keys(30000, seed=8)
+    st, pil, off, bs, order = po.build_keys(hk, lam, P, intermediates=True)
+    assert st == 0
+    np_, B = po.sizes(hk.shape[0], lam, P)
+    for p in range(np_):
+        sizes = bs[p * B:(p + 1) * B]
+        nonempty = [b for b in range(B) if sizes[b] > 0]
+        expect = sorted(nonempty, key=lambda b: (-int(sizes[b]), -b))
+        got = list(order[p * B:p * B + len(expect)])
+        assert got == expect
+        assert all(order[p * B + len(expect):(p + 1) * B] == 0xFFFFFFFF)
+        assert sizes.sum() == off[p + 1] - off[p]
+
+
+def _mix_np(key, pilot):
+    key = (key ^ np.uint32(pilot)).astype(np.uint32)
+    with np.errstate(over="ignore"):
+        key = (((key >> np.uint32(16)) ^ key) * np.uint32(0x45D9F3B)).astype(np.uint32)
+        key = (((key >> np.uint32(16)) ^ key) * np.uint32(0x45D9F3B)).astype(np.uint32)
+    return (key >> np.uint32(16)) ^ key
+
+
+@pytest.mark.parametrize("n,lam,P,seed", [(3000, 4.0, 100, 21), (2500, 3.0, 64, 22), (6000, 7.0, 256, 23)])
+def test_search_minimal_pilots_bruteforce(n, lam, P, seed):
+    """Replays every partition: each stored V is valid and no smaller value is."""
+    hk = _keys(n, seed=seed)
+    st, pil, off, bs, order = po.build_keys(hk, lam, P, intermediates=True)
+    assert st == 0
+    np_, B = po.sizes(n, lam, P)
+    f = po.fulcrums(lam, P)
+    lib = po.load()
+    part = np.array([lib.orc_partition_of(int(k[0]), np_) for k in hk])
+    bkt = np.array([lib.orc_bucket_of(int(k[1]), f.ctypes.data) for k in hk])
+    for p in range(np_):
+        ps = int(off[p + 1] - off[p])
+        taken = np.zeros(ps, dtype=bool)
+        m = int((bs[p * B:(p + 1) * B] > 0).sum())
+        for r in range(m):
+            b = int(order[p * B + r])
+            sel = (part == p) & (bkt == b)
+            l1, l2 = hk[sel, 2], hk[sel, 3]
+            V = int(pil[p + b * np_])
+            found = None
+            for pilot in range(V // ps + 1):
+                pos = (_mix_np(l1, _mix_np(l2, pilot)) >> np.uint32(1)) % np.uint32(ps)
+                if np.unique(pos).size != pos.size:
+                    continue
+                d = np.arange(ps)
+                slots = (pos[None, :].astype(np.int64) + d[:, None]) % ps
+                ok = ~taken[slots].any(axis=1)
+                if ok.any():
+                    found = pilot * ps + int(np.argmax(ok))
+                    break
+            assert found == V, (p, b, found, V)
+            pos = (_mix_np(l1, _mix_np(l2, V // ps)) >> np.uint32(1)) % np.uint32(ps)
+            taken[(pos.astype(np.int64) + V % ps) % ps] = True
+        assert taken.all()
+    _check_bijection(hk, pil, off, lam, P)
+
+
+def test_c1_golden_and_bijection():
+    g = np.load(os.path.join(GOLD, "c1_pilots.npz"))
+    hk = _keys(1_000_000, seed=1)
+    st, pil, off = po.build_keys(hk, 7.0, 2048)
+    assert st == 0
+    assert np.array_equal(pil, g["pilots"])
+    assert np.array_equal(off, g["offsets"])
+    _check_bijection(hk, pil, off, 7.0, 2048)
+
+
+@pytest.mark.parametrize("lam,P", [(8.0, 2500), (7.5, 2048), (6.0, 2048), (4.5, 2500), (12.0, 1024)])
+def test_bijection_configs(lam, P):
+    hk = _keys(300_000, seed=int(lam * 10) + P)
+    st, pil, off = po.build_keys(hk, lam, P)
+    assert st == 0
+    _check_bijection(hk, pil, off, lam, P)
+
+
+def test_empty_and_tiny_inputs():
+    st, pil, off = po.build_keys(np.zeros((0, 4), np.uint32), 7.0, 2048)
+    assert st == 0 and pil.size == 0 and list(off) == [0]
+    for n in (1, 2, 7, 100):
+        hk = _keys(n, seed=40 + n)
+        st, pil, off = po.build_keys(hk, 7.0, 2048)
+        assert st == 0 and int(off[-1]) == n
+        _check_bijection(hk, pil, off, 7.0, 2048)
+
+
+def test_duplicate_keys_detected():
+    hk = _keys(5000, seed=9)
+    hk[4000] = hk[17]
+    st, _, _ = po.build_keys(hk, 7.0, 2048)
+    assert st == 7
+
+
+def test_bucket_overflow_detected():
+    hk = _keys(4000, seed=10)
+    hk[:300, 0] = 5  # partition 0
+    hk[:300, 1] = 12345  # one bucket
+    st, _, _ = po.build_keys(hk, 7.0, 2048)
+    assert st == 6
+
+
+def test_partition_overflow_detected():
+    P = 256
+    hk = _keys(4 * P, seed=12)
+    hk[: P + P // 2 + 1, 0] = 0  # > 1.5 P keys into partition 0
+    st, _, _ = po.build_keys(hk, 7.0, P)
+    assert st == 5
+
+
+def test_compact_vector_known_answer():
+    words, w = po.compact_encode(np.array([1, 2, 3], np.uint64))
+    assert w == 2 and list(words) == [1 | (2 << 2) | (3 << 4), 0]
+    words, w = po.compact_encode(np.array([0, 0, 0], np.uint64))
+    assert w == 1 and words.size == 2 and int(words[0]) == 0
+
+
+@pytest.mark.parametrize("n,maxv", [(1, 1), (64, 2**20), (1000, 2**31 - 1), (333, 5), (77, 2**63)])
+def test_compact_vector_roundtrip(n, maxv):
+    rng = np.random.default_rng(n)
+    vals = rng.integers(0, maxv, n, dtype=np.uint64, endpoint=True)
+    words, w = po.compact_encode(vals)
+    assert w == max(1, int(vals.max()).bit_length())
+    assert words.size == (n * w + 63) // 64 + 1
+    for i in range(n):
+        assert po.compact_access(words, w, i) == int(vals[i])
+
+
+def _parse_mphf(buf):
+    v = np.frombuffer(buf, dtype="<u8")
+    magic, n, P, np_, B = (int(x) for x in v[:5])
+    at = 5
+    fl = v[at:at + 1024].view("<u4")
+    at += 1024
+
+    def compact(at):
+        size, w, mask, nw = (int(x) for x in v[at:at + 4])
+        words = v[at + 4:at + 4 + nw]
+        return [po.compact_access(np.ascontiguousarray(words), w, i) for i in range(size)], at + 4 + nw
+
+    cols = []
+    for _ in range(B):
+        c, at = compact(at)
+        cols.append(c)
+    inc = int(v[at])
+    diffs, at = compact(at + 1)
+    assert at == v.size
+    return magic, n, P, np_, B, fl, cols, inc, diffs
+
+
+def test_encoded_mphf_layout_decodes():
+    lam, P = 7.0, 512
+    hk = _keys(20000, seed=30)
+    st, pil, off = po.build_keys(hk, lam, P)
+    buf = po.encode_mphf(pil, off, lam, P, hk.shape[0])
+    magic, n, P2, np_, B, fl, cols, inc, diffs = _parse_mphf(buf)
+    assert magic == 0x3148504D424F4850 and n == 20000 and P2 == P and inc == P
+    assert np.array_equal(fl, po.fulcrums(lam, P))
+    for j in range(B):
+        assert cols[j] == [int(x) for x in pil[j * np_:(j + 1) * np_]]
+    # linear_diff_encoder: value = (|v| << 1) | (v > 0), v = off[i] - i*P
+    for i, e in enumerate(diffs):
+        s = (2 * (e & 1) - 1) * (e >> 1)
+        assert s + i * P == int(off[i])
