@@ -39,7 +39,8 @@ def stage_bytes_per_key(name: str, lam: float, P: int) -> float:
     s4 = 8 + 1 + 4 / lam + 8
     s5 = 8 + 1028 / P + 16 / lam
     return {
-        "hash_partition_scatter": s1 + s4,
+        "hash_route": s1,               # hash + map; level-1 routing
+        "partition_split": s4,          # routing to partitions (the redistribution)
         "partition_offsets": s3_scan,
         "partition_search": s2 + s3_apply + s5,
     }.get(name, 0.0)
@@ -75,6 +76,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1_000_000_000)
     ap.add_argument("--lam", type=float, default=7.5)
     ap.add_argument("--P", type=int, default=2048)
+    ap.add_argument("--alpha", type=float, default=1.0)
     ap.add_argument("--key-seed", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -86,8 +88,10 @@ def main() -> int:
         return 2
 
     n, lam, P = args.n, args.lam, args.P
-    cfg = ph.make_config(lam, P, 0)
+    alpha = args.alpha
+    cfg = ph.make_config(lam, P, 0, alpha)
     np_, B = ph.sizes(cfg, n)
+    rcap = ph.remap_capacity(cfg, n)
     ctx = ph.Context(0)
     stream = hip_rt.Stream()
     dk = hip_rt.DeviceBuffer(n * 8)
@@ -95,10 +99,13 @@ def main() -> int:
     dp = hip_rt.DeviceBuffer(np_ * B * 4)
     do = hip_rt.DeviceBuffer((np_ + 1) * 4)
     ds = hip_rt.DeviceBuffer(4)
+    drm = hip_rt.DeviceBuffer(rcap * 4) if rcap else None
+    dro = hip_rt.DeviceBuffer((np_ + 1) * 4) if rcap else None
+    mphf = ph.device_outputs(dp, do, drm, dro)
     stream.synchronize()
 
     for _ in range(args.warmup):
-        ctx.build_u64_device(dk, n, cfg, dp, do, ds, stream)
+        ctx.build_u64_device(dk, n, cfg, mphf, ds, stream)
     stream.synchronize()
     status = int(ds.to_numpy(np.uint32)[0])
     if status:
@@ -110,7 +117,7 @@ def main() -> int:
     t0 = time.perf_counter()
     e0.record(stream)
     for _ in range(args.steps):
-        ctx.build_u64_device(dk, n, cfg, dp, do, ds, stream)
+        ctx.build_u64_device(dk, n, cfg, mphf, ds, stream)
         for k, v in ctx.last_stage_times().items():  # waits for this build's events
             stage_sum[k] = stage_sum.get(k, 0.0) + v
     e1.record(stream)
@@ -141,9 +148,10 @@ def main() -> int:
         "dtype": "u64",
         "data": f"synthetic: {n} distinct u64 keys, splitmix64 seed {args.key_seed}, generated in HBM",
         "config": {
-            "workload": f"plain PHOBIC build of {n} u64 keys, lambda {lam}, partition size {P}, alpha 1.0, "
+            "workload": f"plain PHOBIC build of {n} u64 keys, lambda {lam}, partition size {P}, alpha {alpha}, "
                         "XXH3 seed 0, inputs and outputs resident in HBM",
-            "n": n, "lambda": lam, "partition_size": P, "partitions": np_, "buckets_per_partition": B,
+            "n": n, "lambda": lam, "partition_size": P, "alpha": alpha, "partitions": np_,
+            "buckets_per_partition": B,
         },
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
@@ -160,7 +168,7 @@ def main() -> int:
     }
     if not args.no_verify:
         out_pos = hip_rt.DeviceBuffer(n * 4)
-        ctx.eval_u64_device(dk, n, cfg, np_, dp, do, out_pos, stream)
+        ctx.eval_u64_device(dk, n, cfg, np_, mphf, out_pos, stream)
         bm, bad = hip_rt.DeviceBuffer((n + 31) // 32 * 4), hip_rt.DeviceBuffer(8)
         ctx.check_bijection_device(out_pos, n, n, bm, bad, stream)
         stream.synchronize()

@@ -55,7 +55,22 @@ typedef struct {
   uint32_t buckets;           /* 0 = ceil(P / lambda); else B for `fulcrums` */
   uint64_t seed;              /* XXH3 seed */
   const uint32_t* fulcrums;   /* NULL = OptBucketer; else 2048 entries */
+  double alpha;               /* load factor in [0.5, 1]; 0 or 1 = the reference */
 } phobic_config;
+
+/* Build outputs (host or device pointers, as the entry point says).
+ * alpha = 1: pilots and offsets only.  alpha < 1 (an extension with no
+ * reference counterpart): partition p with ps keys searches a table of
+ * m = ceil(ps / alpha) slots (ppm arithmetic); its used slots q in [ps, m)
+ * are remapped, in increasing order, onto the free slots of [0, ps):
+ * remap[remap_offsets[p] + q - ps] (0 for unused q); remap_offsets[np] is
+ * the number of entries used (at most phobic_remap_capacity). */
+typedef struct {
+  uint32_t* pilots;         /* np*B, pilots[p + b*np] */
+  uint32_t* offsets;        /* np+1 */
+  uint32_t* remap;          /* alpha < 1 only */
+  uint32_t* remap_offsets;  /* alpha < 1 only, np+1 */
+} phobic_outputs;
 
 /* The reference's pre-hashed Key (include/phobicGpu/hasher.hpp:7-20). */
 typedef struct {
@@ -86,6 +101,9 @@ const char* phobic_status_string(int status);
 /* np = ceil(n / P), B = ceil(P / lambda)  (mphf_builder.h:90-91, mphf_config.h:29). */
 int phobic_query_sizes(const phobic_config* cfg, uint64_t n, uint32_t* np, uint32_t* buckets);
 
+/* Upper bound of remap entries for n keys (0 when alpha = 1). */
+int phobic_remap_capacity(const phobic_config* cfg, uint64_t n, uint64_t* entries);
+
 /* The 2048-entry fulcrum table (mphf_config.h:44-55, bucketers.h:28-39). */
 int phobic_fulcrums(const phobic_config* cfg, uint32_t* fulcs_out);
 
@@ -97,16 +115,15 @@ int phobic_xxh3_128(const void* data, uint64_t len, uint64_t seed, uint64_t* lo6
 
 /* u64 keys, hashed on the device (hasher.hpp:42-48 semantics). */
 int phobic_build_u64(phobic_ctx* ctx, const uint64_t* keys, uint64_t n, const phobic_config* cfg,
-                     uint32_t* pilots_out, uint32_t* offsets_out, phobic_stage_times* times);
+                     const phobic_outputs* out, phobic_stage_times* times);
 
 /* Byte strings: key i = bytes[offsets[i] .. offsets[i+1]) (hasher.hpp:33-39). */
 int phobic_build_bytes(phobic_ctx* ctx, const uint8_t* bytes, const uint64_t* offsets, uint64_t n,
-                       const phobic_config* cfg, uint32_t* pilots_out, uint32_t* offsets_out,
-                       phobic_stage_times* times);
+                       const phobic_config* cfg, const phobic_outputs* out, phobic_stage_times* times);
 
 /* Pre-hashed keys: the reference's nohash path (mphf_builder.h:202-207). */
 int phobic_build_keys(phobic_ctx* ctx, const phobic_key* keys, uint64_t n, const phobic_config* cfg,
-                      uint32_t* pilots_out, uint32_t* offsets_out, phobic_stage_times* times);
+                      const phobic_outputs* out, phobic_stage_times* times);
 
 /* ---- builds on device-resident buffers (asynchronous on `stream`) ----
  * Inputs and outputs are device pointers.  The call enqueues the whole
@@ -116,14 +133,83 @@ int phobic_build_keys(phobic_ctx* ctx, const phobic_key* keys, uint64_t n, const
  * context pool, so builds on one context must not overlap in time unless
  * they use distinct contexts. */
 int phobic_build_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t n,
-                            const phobic_config* cfg, uint32_t* d_pilots, uint32_t* d_offsets,
-                            uint32_t* d_status, void* stream);
+                            const phobic_config* cfg, const phobic_outputs* d_out, uint32_t* d_status,
+                            void* stream);
 int phobic_build_bytes_device(phobic_ctx* ctx, const uint8_t* d_bytes, const uint64_t* d_offsets,
-                              uint64_t n, const phobic_config* cfg, uint32_t* d_pilots,
-                              uint32_t* d_part_offsets, uint32_t* d_status, void* stream);
+                              uint64_t n, const phobic_config* cfg, const phobic_outputs* d_out,
+                              uint32_t* d_status, void* stream);
 int phobic_build_keys_device(phobic_ctx* ctx, const phobic_key* d_keys, uint64_t n,
-                             const phobic_config* cfg, uint32_t* d_pilots, uint32_t* d_offsets,
-                             uint32_t* d_status, void* stream);
+                             const phobic_config* cfg, const phobic_outputs* d_out, uint32_t* d_status,
+                             void* stream);
+
+/* ---- batched builds: many independent u64 key sets, one configuration ----
+ * set s = d_keys[set_offsets[s] .. set_offsets[s+1]) (set_offsets: host,
+ * sets+1 entries, set_offsets[0] = 0).  All sets' partitions are built as one
+ * flattened device build; every set's pilots and offsets equal those of
+ * building it alone.  Outputs of set s (device):
+ *   pilots  at d_pilots  + B * part_base[s]   (np_s*B, pilots[p + b*np_s])
+ *   offsets at d_offsets + part_base[s] + s   (np_s + 1)
+ * with part_base[s] = sum over s' < s of ceil(n_s' / P) (phobic_batch_layout).
+ * alpha must be 1.  The batch keeps device copies of the set layout; the
+ * device scratch is the context's (one batch or build at a time per context).
+ * phobic_batch_launch enqueues the kernels on `stream`;
+ * phobic_batch_graph_launch captures them into a hipGraph on first use
+ * (and again whenever the device pointers change) and replays it. */
+typedef struct phobic_batch phobic_batch;
+int phobic_batch_create(phobic_ctx* ctx, const phobic_config* cfg, const uint64_t* set_offsets, uint32_t sets,
+                        phobic_batch** out);
+int phobic_batch_layout(const phobic_batch* batch, uint32_t* part_base_out, uint32_t* buckets);
+int phobic_batch_launch(phobic_batch* batch, const uint64_t* d_keys, uint32_t* d_pilots, uint32_t* d_offsets,
+                        uint32_t* d_status, void* stream);
+int phobic_batch_graph_launch(phobic_batch* batch, const uint64_t* d_keys, uint32_t* d_pilots,
+                              uint32_t* d_offsets, uint32_t* d_status, void* stream);
+int phobic_batch_destroy(phobic_batch* batch);
+
+/* ---- serving: continuous-batching build server ----
+ * Requests are admitted in arrival order (a later request may overtake one
+ * that does not fit yet) whenever one of `slots` workers is idle and the
+ * request's device-memory estimate fits `hbm_budget`; each worker owns a
+ * context and stream, so admitted builds run concurrently on the device.
+ * Results stay in device memory until phobic_server_release.  Host key
+ * buffers must stay valid until the request completes.  No counterpart in
+ * the reference (one synchronous MPHFbuilder::build per MPHF). */
+#define PHOBIC_KEYS_U64 0
+#define PHOBIC_KEYS_BYTES 1
+typedef struct {
+  int kind;                 /* PHOBIC_KEYS_U64 or PHOBIC_KEYS_BYTES */
+  int keys_on_device;       /* 0: host pointers, 1: device pointers */
+  const void* keys;         /* u64[n] or the concatenated bytes */
+  const uint64_t* offsets;  /* bytes: n+1 offsets into keys */
+  uint64_t bytes_len;       /* bytes: offsets[n] */
+  uint64_t n;
+  phobic_config cfg;
+} phobic_request;
+typedef struct {
+  uint32_t slots;           /* concurrent builds (1..64) */
+  int device;
+  uint64_t hbm_budget;      /* device bytes requests may hold */
+  uint32_t queue_capacity;  /* 0 = 65536 */
+} phobic_server_config;
+typedef struct {
+  uint64_t submitted, completed, rejected, keys_built, hbm_in_use;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion latency */
+  double span_s;                          /* first submit -> last completion */
+} phobic_server_stats;
+typedef struct phobic_server phobic_server;
+int phobic_server_create(const phobic_server_config* cfg, phobic_server** out);
+/* PHOBIC_E_BUSY when the queue is full, PHOBIC_E_OOM when the request alone
+ * exceeds the budget. */
+int phobic_server_submit(phobic_server* server, const phobic_request* req, uint64_t* id);
+int phobic_server_poll(phobic_server* server, uint64_t id, int* done);
+int phobic_server_wait(phobic_server* server, uint64_t id, int* status, double* latency_ms);
+/* Sizes and device addresses of a completed request's outputs. */
+int phobic_server_result(phobic_server* server, uint64_t id, uint32_t* np, uint32_t* buckets,
+                         uint64_t* remap_entries, phobic_outputs* device_outputs);
+/* Copies a completed request's outputs into host buffers. */
+int phobic_server_fetch(phobic_server* server, uint64_t id, const phobic_outputs* host_out);
+int phobic_server_release(phobic_server* server, uint64_t id);
+int phobic_server_stats_get(phobic_server* server, phobic_server_stats* stats);
+int phobic_server_destroy(phobic_server* server);
 
 /* Stage timings of the last build on this context (device events). */
 int phobic_last_stage_times(phobic_ctx* ctx, phobic_stage_times* times);
@@ -132,8 +218,8 @@ int phobic_last_stage_times(phobic_ctx* ctx, phobic_stage_times* times);
  * Device-side batched query of u64 keys against (pilots, offsets) in HBM:
  * d_out[i] = position of d_keys[i] in [0, n). */
 int phobic_eval_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t count,
-                           const phobic_config* cfg, uint32_t np, const uint32_t* d_pilots,
-                           const uint32_t* d_offsets, uint32_t* d_out, void* stream);
+                           const phobic_config* cfg, uint32_t np, const phobic_outputs* d_mphf,
+                           uint32_t* d_out, void* stream);
 
 /* The validity contract of example.cpp:43-59 / benchmark.cpp:45-62 on the
  * device: d_bad (device u64) receives the number of positions in d_pos that

@@ -318,7 +318,9 @@ static inline uint32_t init_pos(const orc_key* k, uint32_t pilot, uint32_t ps) {
  * smallest V = d + ps*pilot whose positions (init_pos + d) mod ps are
  * pairwise distinct and untaken; then mark them taken. */
 static int search_partition(const orc_key* keys, uint32_t ps, uint32_t nbk, const uint32_t* bucket_list,
-                            const uint32_t* bucket_start, const uint32_t* bucket_size, uint32_t* pilot_of) {
+                            const uint32_t* bucket_start, const uint32_t* bucket_size, uint32_t* pilot_of,
+                            uint64_t** taken_out) {
+  /* ps here is the table size m of the partition (= key count when alpha = 1) */
   uint64_t words = ((uint64_t)ps + 63) / 64;
   uint64_t* taken = (uint64_t*)calloc(words ? words : 1, 8);
   uint64_t* local = (uint64_t*)calloc(words ? words : 1, 8);
@@ -374,9 +376,25 @@ static int search_partition(const orc_key* keys, uint32_t ps, uint32_t nbk, cons
       }
     }
   }
-  free(taken);
+  if (taken_out)
+    *taken_out = taken;
+  else
+    free(taken);
   free(local);
   return status;
+}
+
+/* Table size of a partition with ps keys at load factor alpha = a_ppm / 1e6
+ * (this library's extension; the reference only has alpha = 1):
+ * m = ceil(ps * 1e6 / a_ppm). */
+uint32_t orc_table_size(uint32_t ps, uint32_t a_ppm) {
+  if (a_ppm >= 1000000u || a_ppm == 0) return ps;
+  return (uint32_t)(((uint64_t)ps * 1000000u + a_ppm - 1) / a_ppm);
+}
+
+uint64_t orc_remap_capacity(uint64_t n, uint32_t np, uint32_t a_ppm) {
+  if (a_ppm >= 1000000u || a_ppm == 0) return 0;
+  return (n * (1000000u - a_ppm) + a_ppm - 1) / a_ppm + np;
 }
 
 /* Several errors in one build report the smallest status code. */
@@ -388,7 +406,20 @@ static int merge_err(int a, int b) {
 
 int orc_build_keys(const orc_key* keys, uint64_t n, double lambda, uint32_t P, uint32_t* pilots,
                    uint32_t* offsets, uint32_t* bucket_sizes_out, uint32_t* order_out) {
-  if (P == 0 || !(lambda > 0) || n > 0xFFFFFFFFull) return ORC_E_INVALID;
+  return orc_build_keys_ex(keys, n, lambda, P, 1000000u, pilots, offsets, NULL, NULL, bucket_sizes_out, order_out);
+}
+
+/* Build with load factor alpha = a_ppm / 1e6.  For alpha < 1 partition p
+ * searches a table of m_p = orc_table_size(ps_p) slots; the used slots q in
+ * [ps_p, m_p) are remapped, in increasing order, onto the free slots of
+ * [0, ps_p), also in increasing order: remap[remap_offsets[p] + q - ps_p] is
+ * the slot q stands for (0 for unused q).  remap_offsets[np] = entries. */
+int orc_build_keys_ex(const orc_key* keys, uint64_t n, double lambda, uint32_t P, uint32_t a_ppm, uint32_t* pilots,
+                      uint32_t* offsets, uint32_t* remap, uint32_t* remap_offsets, uint32_t* bucket_sizes_out,
+                      uint32_t* order_out) {
+  if (P == 0 || !(lambda > 0) || n > 0xFFFFFFFFull || a_ppm == 0 || a_ppm > 1000000u) return ORC_E_INVALID;
+  const int use_remap = a_ppm < 1000000u;
+  if (use_remap && (!remap || !remap_offsets)) return ORC_E_INVALID;
   uint32_t np, B;
   orc_sizes(n, lambda, P, &np, &B);
   if (B == 0 || B > 65535) return ORC_E_INVALID;
@@ -452,6 +483,13 @@ int orc_build_keys(const orc_key* keys, uint64_t n, double lambda, uint32_t P, u
   free(fill);
   free(gb);
   memset(pilots, 0, NB * 4); /* empty buckets: pilot 0 */
+  if (use_remap) {
+    remap_offsets[0] = 0;
+    for (uint32_t p = 0; p < np; p++) {
+      const uint32_t ps = offsets[p + 1] - offsets[p];
+      remap_offsets[p + 1] = remap_offsets[p] + (orc_table_size(ps, a_ppm) - ps);
+    }
+  }
   int err = ORC_OK;
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t p = 0; p < (int64_t)np; p++) {
@@ -466,11 +504,27 @@ int orc_build_keys(const orc_key* keys, uint64_t n, double lambda, uint32_t P, u
       sz[j] = cnt[(uint64_t)p * B + ord[j]];
       at += sz[j];
     }
-    int s = search_partition(sorted + offsets[p], offsets[p + 1] - offsets[p], m, ord, st, sz, pv);
+    const uint32_t ps = offsets[p + 1] - offsets[p];
+    const uint32_t mt = orc_table_size(ps, a_ppm);
+    uint64_t* taken = NULL;
+    int s = search_partition(sorted + offsets[p], mt, m, ord, st, sz, pv, use_remap ? &taken : NULL);
     if (s) {
 #pragma omp critical
       err = merge_err(err, s);
     }
+    if (use_remap && !s) {
+      /* k-th used slot of [ps, mt) -> k-th free slot of [0, ps) */
+      uint32_t* out = remap + remap_offsets[p];
+      uint32_t hole = 0;
+      for (uint32_t q = ps; q < mt; q++) {
+        out[q - ps] = 0;
+        if (taken[q >> 6] & (1ull << (q & 63))) {
+          while (taken[hole >> 6] & (1ull << (hole & 63))) hole++;
+          out[q - ps] = hole++;
+        }
+      }
+    }
+    free(taken);
     for (uint32_t j = 0; j < m; j++) pilots[(uint64_t)p + (uint64_t)ord[j] * np] = pv[j];
     free(st);
     free(sz);
@@ -497,6 +551,24 @@ void orc_query_keys(const orc_key* keys, uint64_t n, const uint32_t* pilots, con
     uint32_t hp = V / ps;
     uint32_t hv = orc_mix32(keys[i].lower1, orc_mix32(keys[i].lower2, hp)) >> 1;
     out[i] = off + (hv + V) % ps;
+  }
+}
+
+/* Query for an alpha < 1 build (remap may be NULL when a_ppm = 1e6). */
+void orc_query_keys_ex(const orc_key* keys, uint64_t n, const uint32_t* pilots, const uint32_t* offsets,
+                       uint32_t np, uint32_t B, const uint32_t* fulcs, uint32_t a_ppm, const uint32_t* remap,
+                       const uint32_t* remap_offsets, uint32_t* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    uint32_t p = orc_partition_of(keys[i].partitioner, np);
+    uint32_t b = orc_bucket_of(keys[i].bucketer, fulcs);
+    uint32_t V = pilots[(uint64_t)p + (uint64_t)b * np];
+    uint32_t off = offsets[p], ps = offsets[p + 1] - offsets[p];
+    uint32_t m = orc_table_size(ps, a_ppm);
+    uint32_t hv = orc_mix32(keys[i].lower1, orc_mix32(keys[i].lower2, V / m)) >> 1;
+    uint32_t pos = (hv + V) % m;
+    if (pos >= ps) pos = remap[remap_offsets[p] + pos - ps];
+    out[i] = off + pos;
   }
 }
 

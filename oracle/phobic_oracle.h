@@ -68,6 +68,18 @@ uint32_t orc_mix32(uint32_t key, uint32_t pilot);
 int orc_build_keys(const orc_key* keys, uint64_t n, double lambda, uint32_t partition_size,
                    uint32_t* pilots, uint32_t* offsets, uint32_t* bucket_sizes, uint32_t* order);
 
+/* Load factor extension (no reference counterpart): alpha = a_ppm / 1e6,
+ * table size m = ceil(ps * 1e6 / a_ppm); used slots of [ps, m) remapped in
+ * order onto the free slots of [0, ps).  a_ppm = 1e6 is the reference. */
+uint32_t orc_table_size(uint32_t ps, uint32_t a_ppm);
+uint64_t orc_remap_capacity(uint64_t n, uint32_t np, uint32_t a_ppm);
+int orc_build_keys_ex(const orc_key* keys, uint64_t n, double lambda, uint32_t partition_size, uint32_t a_ppm,
+                      uint32_t* pilots, uint32_t* offsets, uint32_t* remap, uint32_t* remap_offsets,
+                      uint32_t* bucket_sizes, uint32_t* order);
+void orc_query_keys_ex(const orc_key* keys, uint64_t n, const uint32_t* pilots, const uint32_t* offsets,
+                       uint32_t np, uint32_t B, const uint32_t* fulcs, uint32_t a_ppm, const uint32_t* remap,
+                       const uint32_t* remap_offsets, uint32_t* out);
+
 /* Sizes for a build: np = ceil(n/P), B = ceil(P/lambda). */
 void orc_sizes(uint64_t n, double lambda, uint32_t partition_size, uint32_t* np, uint32_t* B);
 

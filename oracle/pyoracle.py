@@ -37,6 +37,10 @@ def load() -> ctypes.CDLL:
         "orc_bucket_of": (U32, [U32, P]),
         "orc_mix32": (U32, [U32, U32]),
         "orc_build_keys": (ctypes.c_int, [P, U64, D, U32, P, P, P, P]),
+        "orc_build_keys_ex": (ctypes.c_int, [P, U64, D, U32, U32, P, P, P, P, P, P]),
+        "orc_query_keys_ex": (None, [P, U64, P, P, U32, U32, P, U32, P, P, P]),
+        "orc_table_size": (U32, [U32, U32]),
+        "orc_remap_capacity": (U64, [U64, U32, U32]),
         "orc_sizes": (None, [U64, D, U32, P, P]),
         "orc_query_keys": (None, [P, U64, P, P, U32, U32, P, P]),
         "orc_compact_words": (U64, [U64, U64]),
@@ -121,6 +125,42 @@ def build_keys(keys: np.ndarray, lam: float, P: int, intermediates: bool = False
     if intermediates:
         return st, pilots, offsets, bs, order
     return st, pilots, offsets
+
+
+def alpha_ppm(alpha: float) -> int:
+    return int(round(alpha * 1_000_000))
+
+
+def build_keys_alpha(keys: np.ndarray, lam: float, P: int, alpha: float):
+    """alpha <= 1 build: (status, pilots, offsets, remap, remap_offsets)."""
+    lib = load()
+    keys = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1, 4)
+    n = keys.shape[0]
+    np_, B = sizes(n, lam, P)
+    a = alpha_ppm(alpha)
+    pilots = np.zeros(np_ * B, dtype=np.uint32)
+    offsets = np.zeros(np_ + 1, dtype=np.uint32)
+    remap = np.zeros(max(1, lib.orc_remap_capacity(n, np_, a)), dtype=np.uint32)
+    roff = np.zeros(np_ + 1, dtype=np.uint32)
+    kp = keys.ctypes.data if n else np.zeros(4, np.uint32).ctypes.data
+    st = lib.orc_build_keys_ex(kp, n, lam, P, a, pilots.ctypes.data, offsets.ctypes.data, remap.ctypes.data,
+                               roff.ctypes.data, None, None)
+    used = int(roff[-1]) if a < 1_000_000 else 0
+    return st, pilots, offsets, remap[:used], roff
+
+
+def query_keys_alpha(keys, pilots, offsets, remap, roff, lam, P, alpha):
+    lib = load()
+    keys = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1, 4)
+    np_ = offsets.size - 1
+    B = pilots.size // max(np_, 1)
+    f = fulcrums(lam, P)
+    out = np.zeros(keys.shape[0], dtype=np.uint32)
+    rm = np.ascontiguousarray(remap, dtype=np.uint32) if remap.size else np.zeros(1, np.uint32)
+    lib.orc_query_keys_ex(keys.ctypes.data, keys.shape[0], np.ascontiguousarray(pilots).ctypes.data,
+                          np.ascontiguousarray(offsets).ctypes.data, np_, B, f.ctypes.data, alpha_ppm(alpha),
+                          rm.ctypes.data, np.ascontiguousarray(roff).ctypes.data, out.ctypes.data)
+    return out
 
 
 def query_keys(keys: np.ndarray, pilots: np.ndarray, offsets: np.ndarray, lam: float, P: int) -> np.ndarray:

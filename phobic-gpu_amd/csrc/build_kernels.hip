@@ -40,7 +40,7 @@ __device__ __forceinline__ void flag_error(uint32_t* status, uint32_t code) {
 // chunks.  offsets[0] = 0, offsets[p+1] = sum of counts[0..p].
 
 __global__ __launch_bounds__(1024) void k_partition_offsets(const uint32_t* __restrict__ counts, uint32_t np,
-                                                            uint32_t* __restrict__ offsets) {
+                                                            uint32_t a_ppm, uint32_t* __restrict__ offsets) {
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t carry_s;
   const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -56,7 +56,8 @@ __global__ __launch_bounds__(1024) void k_partition_offsets(const uint32_t* __re
 #pragma unroll
     for (uint32_t j = 0; j < kPer; j++) {
       const uint64_t idx = c0 + (uint64_t)tid * kPer + j;
-      v[j] = idx < np ? counts[idx] : 0;
+      const uint32_t c = idx < np ? counts[idx] : 0;
+      v[j] = a_ppm ? table_size(c, a_ppm) - c : c;
       local += v[j];
     }
     // wave inclusive scan of per-thread totals
@@ -104,9 +105,12 @@ __global__ __launch_bounds__(256) void k_eval_u64(EvalArgs a) {
     const uint32_t V = a.pilots[p + (uint64_t)b * a.np];
     const uint32_t off = a.offsets[p];
     const uint32_t ps = a.offsets[p + 1] - off;
-    const uint32_t hp = V / ps;
+    const uint32_t tm = table_size(ps, a.a_ppm);
+    const uint32_t hp = V / tm;
     const uint32_t hv = mix32((uint32_t)h.hi, mix32((uint32_t)(h.hi >> 32), hp)) >> 1;
-    a.out[i] = off + (hv + V) % ps;
+    uint32_t pos = (hv + V) % tm;
+    if (pos >= ps) pos = a.remap[a.remap_offsets[p] + pos - ps];
+    a.out[i] = off + pos;
   }
 }
 
@@ -127,6 +131,24 @@ __global__ __launch_bounds__(256) void k_check_bijection(const uint32_t* __restr
     if (atomicOr(&bitmap[q >> 5], m) & m) local++;
   }
   if (local) atomicAdd(bad, (unsigned long long)local);
+}
+
+// Batched builds: set s owns global partitions [pb[s], pb[s+1]) and its
+// np_s + 1 exclusive offsets start at out[pb[s] + s].
+__global__ __launch_bounds__(256) void k_batch_offsets(const uint32_t* __restrict__ goff,
+                                                       const uint32_t* __restrict__ pb, uint32_t sets,
+                                                       uint32_t total, uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  // set s with pb[s] + s <= t < pb[s+1] + s + 1
+  uint32_t lo = 0, hi = sets;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (pb[mid] + mid <= t) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t j = t - (pb[lo] + lo);
+  out[t] = goff[pb[lo] + j] - goff[pb[lo]];
 }
 
 // splitmix64 (Steele, Lea, Flood 2014): output i uses state seed + (i+1)*gamma.
@@ -155,14 +177,23 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_t* offsets, hipStream_t s) {
-  hipLaunchKernelGGL(k_partition_offsets, dim3(1), dim3(1024), 0, s, counts, np, offsets);
+hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_t a_ppm, uint32_t* offsets,
+                                    hipStream_t s) {
+  hipLaunchKernelGGL(k_partition_offsets, dim3(1), dim3(1024), 0, s, counts, np, a_ppm, offsets);
   return hipGetLastError();
 }
 
 hipError_t launch_eval_u64(const EvalArgs& a, hipStream_t s) {
   if (a.count == 0) return hipSuccess;
   hipLaunchKernelGGL(k_eval_u64, dim3(stream_grid(a.count, 256, 256 * 32)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_offsets(const uint32_t* global_offsets, const uint32_t* set_part_base, uint32_t sets,
+                                uint32_t total_entries, uint32_t* out, hipStream_t s) {
+  if (total_entries == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_offsets, dim3((total_entries + 255) / 256), dim3(256), 0, s, global_offsets,
+                     set_part_base, sets, total_entries, out);
   return hipGetLastError();
 }
 

@@ -12,8 +12,10 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "../../include/phobic_hip.h"
+#include "internal.hpp"
 #include "kernels.hpp"
 
 using namespace phobic;
@@ -59,7 +61,7 @@ struct phobic_ctx {
   // scratch owned by the context
   DevBuf pcount, lvl_count, rec_lo[2], rec_meta[2], fulcs, status;
   // staging for host-buffer entry points
-  DevBuf in_a, in_b, out_pilots, out_offsets;
+  DevBuf in_a, in_b, out_pilots, out_offsets, out_remap, out_roff;
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
   // fulcrums currently in ctx->fulcs
@@ -85,6 +87,8 @@ int hip_status(hipError_t e) { return e == hipSuccess ? PHOBIC_OK : (e == hipErr
 struct Plan {
   uint64_t n;
   uint32_t np, B, cap;
+  uint32_t a_ppm;  // load factor in ppm, 1e6 = alpha 1
+  uint32_t tcap;   // largest table size: table_size(cap, a_ppm)
   // partitioning levels: level l (1..L) bins cover G[l] partitions, G[L] = 1
   uint32_t L, fanout;
   uint64_t G[kMaxLevels + 1];
@@ -128,7 +132,11 @@ int validate(const phobic_config* cfg, uint64_t n, Plan* pl) {
   pl->B = (uint32_t)Bd;
   pl->np = (uint32_t)((n + cfg->partition_size - 1) / cfg->partition_size);
   pl->cap = cfg->partition_size + cfg->partition_size / 2;  // mphf_config.h:61-64
-  if (search_lds_bytes(pl->cap, pl->B) > 64 * 1024) return PHOBIC_E_INVALID_CONFIG;
+  const double alpha = cfg->alpha == 0.0 ? 1.0 : cfg->alpha;
+  if (!(alpha >= 0.5 && alpha <= 1.0)) return PHOBIC_E_INVALID_CONFIG;
+  pl->a_ppm = (uint32_t)std::llround(alpha * 1e6);
+  pl->tcap = table_size(pl->cap, pl->a_ppm);
+  if (search_lds_bytes(pl->tcap, pl->B) > 64 * 1024) return PHOBIC_E_INVALID_CONFIG;
   if (pl->np > (1u << 21)) return PHOBIC_E_INVALID_CONFIG;  // partition index inside a bin must fit 16 bits
   plan_levels(pl);
   if (pl->L > kMaxLevels || pl->G[1] > 65535) return PHOBIC_E_INVALID_CONFIG;
@@ -176,34 +184,66 @@ int set_device(phobic_ctx* ctx) {
   return PHOBIC_OK;
 }
 
-// Enqueue one build.  Inputs/outputs are device pointers.
-int enqueue_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_config* cfg, const Plan& pl,
-                  uint32_t* d_pilots, uint32_t* d_offsets, uint32_t* d_status, hipStream_t s) {
-  TRY(upload_fulcrums(ctx, cfg, s));
-  const uint32_t np = pl.np, L = pl.L, f = pl.fanout;
-  const uint32_t bins1 = (uint32_t)((np + pl.G[1] - 1) / pl.G[1]);
-  const uint64_t slots = (uint64_t)bins1 * pl.G[1] * pl.cap;
+// Batched builds: per-set key ranges and partition bases on the device.
+struct BatchInfo {
+  uint32_t sets = 0;
+  const uint64_t* set_key_off = nullptr;
+  const uint32_t* set_part_base = nullptr;
+  uint32_t* global_offsets = nullptr;  // np + 1 scratch for the per-set offsets
+};
+
+// Layout of the context scratch for a plan.
+struct Layout {
+  uint32_t bins1;
+  uint64_t slots;
+  uint64_t cnt_off[kMaxLevels + 1];
+  uint64_t cnt_total;
+};
+
+Layout layout_of(const Plan& pl) {
+  Layout ly{};
+  ly.bins1 = (uint32_t)((pl.np + pl.G[1] - 1) / pl.G[1]);
+  ly.slots = (uint64_t)ly.bins1 * pl.G[1] * pl.cap;
   // per-level bin counters: level l has ceil(np / G[l-1]) * f bins (level 1: bins1)
-  uint64_t cnt_off[kMaxLevels + 1] = {}, cnt_total = 0;
-  for (uint32_t l = 1; l < L; l++) {
-    cnt_off[l] = cnt_total;
-    cnt_total += l == 1 ? bins1 : ((np + pl.G[l - 1] - 1) / pl.G[l - 1]) * (uint64_t)f;
+  for (uint32_t l = 1; l < pl.L; l++) {
+    ly.cnt_off[l] = ly.cnt_total;
+    ly.cnt_total += l == 1 ? ly.bins1 : ((pl.np + pl.G[l - 1] - 1) / pl.G[l - 1]) * (uint64_t)pl.fanout;
   }
-  TRY_HIP(ctx->pcount.ensure(((uint64_t)np + f) * 4));
-  TRY_HIP(ctx->lvl_count.ensure((cnt_total + 1) * 4));
+  return ly;
+}
+
+// Allocations and the fulcrum upload: everything a build needs before its
+// kernels are enqueued (not capturable into a graph).
+int prepare_build(phobic_ctx* ctx, const phobic_config* cfg, const Plan& pl, hipStream_t s) {
+  TRY(upload_fulcrums(ctx, cfg, s));
+  const Layout ly = layout_of(pl);
+  TRY_HIP(ctx->pcount.ensure(((uint64_t)pl.np + pl.fanout) * 4));
+  TRY_HIP(ctx->lvl_count.ensure((ly.cnt_total + 1) * 4));
   for (int i = 0; i < 2; i++) {
-    TRY_HIP(ctx->rec_lo[i].ensure(slots * 8));
-    TRY_HIP(ctx->rec_meta[i].ensure(slots * 4));
+    TRY_HIP(ctx->rec_lo[i].ensure(ly.slots * 8));
+    TRY_HIP(ctx->rec_meta[i].ensure(ly.slots * 4));
   }
+  return PHOBIC_OK;
+}
+
+// Kernel launches of one build (fills included, no allocation, no host
+// copies): capturable into a hipGraph when `timed` is false.
+int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_config* cfg, const Plan& pl,
+                 const phobic_outputs& d_out, uint32_t* d_status, hipStream_t s, const BatchInfo& batch, bool timed) {
+  uint32_t* d_pilots = d_out.pilots;
+  const bool remap = pl.a_ppm < 1000000u;
+  const uint32_t np = pl.np, L = pl.L, f = pl.fanout;
+  const Layout ly = layout_of(pl);
+  const uint32_t bins1 = ly.bins1;
   TRY_HIP(launch_fill_u32(d_status, 1, 0, s));
   TRY_HIP(launch_fill_u32(ctx->pcount.as<uint32_t>(), (uint64_t)np + f, 0, s));
-  TRY_HIP(launch_fill_u32(ctx->lvl_count.as<uint32_t>(), cnt_total, 0, s));
-  TRY_HIP(hipEventRecord(ctx->ev[0], s));
+  TRY_HIP(launch_fill_u32(ctx->lvl_count.as<uint32_t>(), ly.cnt_total, 0, s));
+  if (timed) TRY_HIP(hipEventRecord(ctx->ev[0], s));
 
   // level l writes buffer 0 when L - l is even, so level L lands in buffer 0
   auto out_buf = [&](uint32_t l) { return (int)((L - l) & 1); };
   auto counts = [&](uint32_t l) {
-    return l == L ? ctx->pcount.as<uint32_t>() : ctx->lvl_count.as<uint32_t>() + cnt_off[l];
+    return l == L ? ctx->pcount.as<uint32_t>() : ctx->lvl_count.as<uint32_t>() + ly.cnt_off[l];
   };
   HashRouteArgs ha{};
   ha.src = src;
@@ -218,8 +258,11 @@ int enqueue_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_
   ha.out_lo = ctx->rec_lo[out_buf(1)].as<uint64_t>();
   ha.out_meta = ctx->rec_meta[out_buf(1)].as<uint32_t>();
   ha.status = d_status;
+  ha.sets = batch.sets;
+  ha.set_key_off = batch.set_key_off;
+  ha.set_part_base = batch.set_part_base;
   TRY_HIP(launch_hash_route(kind, ha, s));
-  TRY_HIP(hipEventRecord(ctx->ev[1], s));
+  if (timed) TRY_HIP(hipEventRecord(ctx->ev[1], s));
 
   for (uint32_t l = 2; l <= L; l++) {
     SplitArgs sp{};
@@ -238,10 +281,17 @@ int enqueue_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_
     sp.status = d_status;
     TRY_HIP(launch_split_level(sp, parents, s));
   }
-  TRY_HIP(hipEventRecord(ctx->ev[2], s));
+  if (timed) TRY_HIP(hipEventRecord(ctx->ev[2], s));
 
-  TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, d_offsets, s));
-  TRY_HIP(hipEventRecord(ctx->ev[3], s));
+  if (batch.sets) {
+    TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, 0, batch.global_offsets, s));
+    TRY_HIP(launch_batch_offsets(batch.global_offsets, batch.set_part_base, batch.sets, np + batch.sets,
+                                 d_out.offsets, s));
+  } else {
+    TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, 0, d_out.offsets, s));
+  }
+  if (remap) TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, pl.a_ppm, d_out.remap_offsets, s));
+  if (timed) TRY_HIP(hipEventRecord(ctx->ev[3], s));
 
   SearchArgs ra{};
   ra.rec_lower = ctx->rec_lo[0].as<uint64_t>();
@@ -251,12 +301,27 @@ int enqueue_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_
   ra.np = np;
   ra.B = pl.B;
   ra.cap = pl.cap;
+  ra.a_ppm = pl.a_ppm;
+  ra.tcap = pl.tcap;
   ra.pilots = d_pilots;
+  ra.remap = remap ? d_out.remap : nullptr;
+  ra.remap_offsets = remap ? d_out.remap_offsets : nullptr;
   ra.status = d_status;
+  ra.sets = batch.sets;
+  ra.set_part_base = batch.set_part_base;
   TRY_HIP(launch_partition_search(ra, s));
-  TRY_HIP(hipEventRecord(ctx->ev[4], s));
-  ctx->timed = true;
+  if (timed) {
+    TRY_HIP(hipEventRecord(ctx->ev[4], s));
+    ctx->timed = true;
+  }
   return PHOBIC_OK;
+}
+
+// Enqueue one build.  Inputs/outputs are device pointers.
+int enqueue_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_config* cfg, const Plan& pl,
+                  const phobic_outputs& d_out, uint32_t* d_status, hipStream_t s) {
+  TRY(prepare_build(ctx, cfg, pl, s));
+  return launch_build(ctx, kind, src, cfg, pl, d_out, d_status, s, BatchInfo{}, true);
 }
 
 int status_from_bits(uint32_t bits) {
@@ -281,13 +346,24 @@ int fill_times(phobic_ctx* ctx, phobic_stage_times* t) {
   return PHOBIC_OK;
 }
 
+uint64_t remap_capacity(const Plan& pl) {
+  if (pl.a_ppm >= 1000000u) return 0;
+  return (pl.n * (1000000u - pl.a_ppm) + pl.a_ppm - 1) / pl.a_ppm + pl.np;
+}
+
+bool outputs_ok(const phobic_outputs* out, const Plan& pl) {
+  if (!out || !out->pilots || !out->offsets) return false;
+  if (pl.a_ppm < 1000000u && (!out->remap || !out->remap_offsets)) return false;
+  return true;
+}
+
 // Host-buffer build: stage inputs, enqueue on the context stream, copy out.
 int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const void* b, size_t b_bytes,
-               uint64_t n, const phobic_config* cfg, uint32_t* pilots_out, uint32_t* offsets_out,
-               phobic_stage_times* times) {
-  if (!ctx || !pilots_out || !offsets_out) return PHOBIC_E_INVALID_CONFIG;
+               uint64_t n, const phobic_config* cfg, const phobic_outputs* out, phobic_stage_times* times) {
+  if (!ctx) return PHOBIC_E_INVALID_CONFIG;
   Plan pl;
   TRY(validate(cfg, n, &pl));
+  if (!outputs_ok(out, pl)) return PHOBIC_E_INVALID_CONFIG;
   TRY(set_device(ctx));
   hipStream_t s = ctx->stream;
   TRY_HIP(ctx->in_a.ensure(a_bytes));
@@ -297,8 +373,13 @@ int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const v
     TRY_HIP(hipMemcpyAsync(ctx->in_b.ptr, b, b_bytes, hipMemcpyHostToDevice, s));
   }
   const uint64_t NB = (uint64_t)pl.np * pl.B;
+  const bool remap = pl.a_ppm < 1000000u;
   TRY_HIP(ctx->out_pilots.ensure(NB * 4 + 4));  // + status word
   TRY_HIP(ctx->out_offsets.ensure(((uint64_t)pl.np + 1) * 4));
+  if (remap) {
+    TRY_HIP(ctx->out_remap.ensure(remap_capacity(pl) * 4));
+    TRY_HIP(ctx->out_roff.ensure(((uint64_t)pl.np + 1) * 4));
+  }
   uint32_t* d_status = ctx->out_pilots.as<uint32_t>() + NB;
   KeySource src{};
   if (kind == kKindU64) src.u64 = ctx->in_a.as<uint64_t>();
@@ -307,29 +388,220 @@ int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const v
     src.offs = ctx->in_b.as<uint64_t>();
   }
   if (kind == kKindKey) src.keys = ctx->in_a.as<uint4>();
-  TRY(enqueue_build(ctx, kind, src, cfg, pl, ctx->out_pilots.as<uint32_t>(), ctx->out_offsets.as<uint32_t>(),
-                    d_status, s));
+  phobic_outputs d_out{ctx->out_pilots.as<uint32_t>(), ctx->out_offsets.as<uint32_t>(),
+                       remap ? ctx->out_remap.as<uint32_t>() : nullptr, remap ? ctx->out_roff.as<uint32_t>() : nullptr};
+  TRY(enqueue_build(ctx, kind, src, cfg, pl, d_out, d_status, s));
   uint32_t bits = 0;
   TRY_HIP(hipMemcpyAsync(&bits, d_status, 4, hipMemcpyDeviceToHost, s));
-  if (NB) TRY_HIP(hipMemcpyAsync(pilots_out, ctx->out_pilots.ptr, NB * 4, hipMemcpyDeviceToHost, s));
-  TRY_HIP(hipMemcpyAsync(offsets_out, ctx->out_offsets.ptr, ((uint64_t)pl.np + 1) * 4, hipMemcpyDeviceToHost, s));
+  if (NB) TRY_HIP(hipMemcpyAsync(out->pilots, ctx->out_pilots.ptr, NB * 4, hipMemcpyDeviceToHost, s));
+  TRY_HIP(hipMemcpyAsync(out->offsets, ctx->out_offsets.ptr, ((uint64_t)pl.np + 1) * 4, hipMemcpyDeviceToHost, s));
+  if (remap)
+    TRY_HIP(hipMemcpyAsync(out->remap_offsets, ctx->out_roff.ptr, ((uint64_t)pl.np + 1) * 4, hipMemcpyDeviceToHost, s));
   TRY_HIP(hipStreamSynchronize(s));
+  if (remap && out->remap_offsets[pl.np]) {
+    TRY_HIP(hipMemcpy(out->remap, ctx->out_remap.ptr, (uint64_t)out->remap_offsets[pl.np] * 4, hipMemcpyDeviceToHost));
+  }
   TRY(fill_times(ctx, times));
   return status_from_bits(bits);
 }
 
 int device_build(phobic_ctx* ctx, int kind, const KeySource& src, uint64_t n, const phobic_config* cfg,
-                 uint32_t* d_pilots, uint32_t* d_offsets, uint32_t* d_status, void* stream) {
-  if (!ctx || !d_pilots || !d_offsets || !d_status) return PHOBIC_E_INVALID_CONFIG;
+                 const phobic_outputs* d_out, uint32_t* d_status, void* stream) {
+  if (!ctx || !d_status) return PHOBIC_E_INVALID_CONFIG;
   Plan pl;
   TRY(validate(cfg, n, &pl));
+  if (!outputs_ok(d_out, pl)) return PHOBIC_E_INVALID_CONFIG;
   TRY(set_device(ctx));
-  return enqueue_build(ctx, kind, src, cfg, pl, d_pilots, d_offsets, d_status, (hipStream_t)stream);
+  return enqueue_build(ctx, kind, src, cfg, pl, *d_out, d_status, (hipStream_t)stream);
+}
+
+}  // namespace
+
+hipStream_t phobic::ctx_stream(phobic_ctx* ctx) { return ctx->stream; }
+
+// A batch of independent key sets built together (one configuration).
+struct phobic_batch {
+  phobic_ctx* ctx = nullptr;
+  phobic_config cfg{};
+  std::vector<uint32_t> fulcs_copy;  // keeps cfg.fulcrums valid
+  Plan pl{};
+  uint32_t sets = 0;
+  std::vector<uint32_t> part_base;  // sets + 1
+  DevBuf d_key_off, d_part_base, d_goff;
+  // graph captured for one set of device pointers
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* g_keys = nullptr;
+  const void* g_pilots = nullptr;
+  const void* g_offsets = nullptr;
+  const void* g_status = nullptr;
+  const void* g_scratch[7] = {};  // context scratch the graph's kernels use
+};
+
+namespace {
+
+// Context scratch addresses baked into a captured graph.
+void scratch_ptrs(const phobic_ctx* ctx, const void* (&p)[7]) {
+  const void* v[7] = {ctx->pcount.ptr, ctx->lvl_count.ptr, ctx->rec_lo[0].ptr, ctx->rec_lo[1].ptr,
+                      ctx->rec_meta[0].ptr, ctx->rec_meta[1].ptr, ctx->fulcs.ptr};
+  for (int i = 0; i < 7; i++) p[i] = v[i];
+}
+
+void batch_drop_graph(phobic_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int batch_enqueue(phobic_batch* b, const uint64_t* d_keys, uint32_t* d_pilots, uint32_t* d_offsets,
+                  uint32_t* d_status, hipStream_t s) {
+  KeySource src{};
+  src.u64 = d_keys;
+  BatchInfo bi;
+  bi.sets = b->sets;
+  bi.set_key_off = b->d_key_off.as<uint64_t>();
+  bi.set_part_base = b->d_part_base.as<uint32_t>();
+  bi.global_offsets = b->d_goff.as<uint32_t>();
+  phobic_outputs out{d_pilots, d_offsets, nullptr, nullptr};
+  return launch_build(b->ctx, kKindU64, src, &b->cfg, b->pl, out, d_status, s, bi, false);
 }
 
 }  // namespace
 
 extern "C" {
+
+int phobic_batch_create(phobic_ctx* ctx, const phobic_config* cfg, const uint64_t* set_offsets, uint32_t sets,
+                        phobic_batch** out) {
+  if (!ctx || !cfg || !set_offsets || !out || sets == 0) return PHOBIC_E_INVALID_CONFIG;
+  *out = nullptr;
+  const double alpha = cfg->alpha == 0.0 ? 1.0 : cfg->alpha;
+  if (alpha != 1.0) return PHOBIC_E_INVALID_CONFIG;  // batches use the reference load factor
+  Plan one;
+  TRY(validate(cfg, 0, &one));
+  std::vector<uint32_t> pb(sets + 1, 0);
+  uint64_t total_parts = 0;
+  for (uint32_t s = 0; s < sets; s++) {
+    if (set_offsets[s + 1] < set_offsets[s]) return PHOBIC_E_INVALID_CONFIG;
+    const uint64_t ns = set_offsets[s + 1] - set_offsets[s];
+    if (ns > 0xFFFFFFFFull) return PHOBIC_E_INVALID_CONFIG;
+    total_parts += (ns + cfg->partition_size - 1) / cfg->partition_size;
+    if (total_parts > 0xFFFFFFFFull) return PHOBIC_E_INVALID_CONFIG;
+    pb[s + 1] = (uint32_t)total_parts;
+  }
+  const uint64_t n = set_offsets[sets] - set_offsets[0];
+  if (set_offsets[0] != 0) return PHOBIC_E_INVALID_CONFIG;
+  // the flattened build: np = total partitions (validate sizes it from n)
+  Plan pl;
+  {
+    phobic_config c2 = *cfg;
+    TRY(validate(&c2, 0, &pl));
+    pl.n = n;
+    pl.np = (uint32_t)total_parts;
+    if (pl.np > (1u << 21)) return PHOBIC_E_INVALID_CONFIG;
+    plan_levels(&pl);
+    if (pl.L > kMaxLevels || pl.G[1] > 65535) return PHOBIC_E_INVALID_CONFIG;
+  }
+  phobic_batch* b = new (std::nothrow) phobic_batch();
+  if (!b) return PHOBIC_E_OOM;
+  b->ctx = ctx;
+  b->cfg = *cfg;
+  if (cfg->fulcrums) {
+    b->fulcs_copy.assign(cfg->fulcrums, cfg->fulcrums + kFulcsTable);
+    b->cfg.fulcrums = b->fulcs_copy.data();
+  }
+  b->pl = pl;
+  b->sets = sets;
+  b->part_base = pb;
+  int st = set_device(ctx);
+  hipStream_t s = ctx->stream;
+  auto fail = [&](int code) {
+    b->d_key_off.release();
+    b->d_part_base.release();
+    b->d_goff.release();
+    delete b;
+    return code;
+  };
+  if (st) return fail(st);
+  hipError_t e = b->d_key_off.ensure((uint64_t)(sets + 1) * 8);
+  if (e == hipSuccess) e = b->d_part_base.ensure((uint64_t)(sets + 1) * 4);
+  if (e == hipSuccess) e = b->d_goff.ensure(((uint64_t)pl.np + 1) * 4);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_key_off.ptr, set_offsets, (uint64_t)(sets + 1) * 8, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_part_base.ptr, pb.data(), (uint64_t)(sets + 1) * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(hip_status(e));
+  st = prepare_build(ctx, &b->cfg, pl, s);
+  if (st) return fail(st);
+  *out = b;
+  return PHOBIC_OK;
+}
+
+int phobic_batch_layout(const phobic_batch* b, uint32_t* part_base_out, uint32_t* buckets) {
+  if (!b) return PHOBIC_E_INVALID_CONFIG;
+  if (part_base_out) std::memcpy(part_base_out, b->part_base.data(), (b->sets + 1) * 4);
+  if (buckets) *buckets = b->pl.B;
+  return PHOBIC_OK;
+}
+
+int phobic_batch_launch(phobic_batch* b, const uint64_t* d_keys, uint32_t* d_pilots, uint32_t* d_offsets,
+                        uint32_t* d_status, void* stream) {
+  if (!b || !d_pilots || !d_offsets || !d_status || (b->pl.n && !d_keys)) return PHOBIC_E_INVALID_CONFIG;
+  TRY(set_device(b->ctx));
+  // the context scratch may have been resized by another build in between
+  TRY(prepare_build(b->ctx, &b->cfg, b->pl, (hipStream_t)stream));
+  return batch_enqueue(b, d_keys, d_pilots, d_offsets, d_status, (hipStream_t)stream);
+}
+
+int phobic_batch_graph_launch(phobic_batch* b, const uint64_t* d_keys, uint32_t* d_pilots, uint32_t* d_offsets,
+                              uint32_t* d_status, void* stream) {
+  if (!b || !d_pilots || !d_offsets || !d_status || (b->pl.n && !d_keys)) return PHOBIC_E_INVALID_CONFIG;
+  TRY(set_device(b->ctx));
+  hipStream_t s = (hipStream_t)stream;
+  // fulcrums re-uploaded and scratch grown as needed; a replay is valid
+  // only while every address the graph captured is unchanged
+  TRY(prepare_build(b->ctx, &b->cfg, b->pl, s));
+  const void* cur[7];
+  scratch_ptrs(b->ctx, cur);
+  bool same = b->exec && b->g_keys == d_keys && b->g_pilots == d_pilots && b->g_offsets == d_offsets &&
+              b->g_status == d_status;
+  for (int i = 0; i < 7 && same; i++) same = cur[i] == b->g_scratch[i];
+  if (!same) {
+    batch_drop_graph(b);
+    hipStream_t cs = nullptr;
+    TRY_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int st = e == hipSuccess ? batch_enqueue(b, d_keys, d_pilots, d_offsets, d_status, cs) : hip_status(e);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (st != PHOBIC_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    TRY_HIP(e2);
+    b->graph = g;
+    TRY_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->g_keys = d_keys;
+    b->g_pilots = d_pilots;
+    b->g_offsets = d_offsets;
+    b->g_status = d_status;
+    for (int i = 0; i < 7; i++) b->g_scratch[i] = cur[i];
+  }
+  TRY_HIP(hipGraphLaunch(b->exec, s));
+  return PHOBIC_OK;
+}
+
+int phobic_batch_destroy(phobic_batch* b) {
+  if (!b) return PHOBIC_OK;
+  (void)hipSetDevice(b->ctx->device);
+  batch_drop_graph(b);
+  b->d_key_off.release();
+  b->d_part_base.release();
+  b->d_goff.release();
+  delete b;
+  return PHOBIC_OK;
+}
+
 
 int phobic_ctx_create(int device, phobic_ctx** out) {
   if (!out) return PHOBIC_E_INVALID_CONFIG;
@@ -356,7 +628,7 @@ int phobic_ctx_destroy(phobic_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (DevBuf* b : {&ctx->pcount, &ctx->lvl_count, &ctx->rec_lo[0], &ctx->rec_lo[1], &ctx->rec_meta[0], &ctx->rec_meta[1], &ctx->fulcs, &ctx->status, &ctx->in_a, &ctx->in_b,
-                    &ctx->out_pilots, &ctx->out_offsets})
+                    &ctx->out_pilots, &ctx->out_offsets, &ctx->out_remap, &ctx->out_roff})
     b->release();
   for (int i = 0; i <= kStages; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
@@ -398,53 +670,59 @@ int phobic_fulcrums(const phobic_config* cfg, uint32_t* out) {
   return PHOBIC_OK;
 }
 
+int phobic_remap_capacity(const phobic_config* cfg, uint64_t n, uint64_t* entries) {
+  Plan pl;
+  TRY(validate(cfg, n, &pl));
+  if (!entries) return PHOBIC_E_INVALID_CONFIG;
+  *entries = remap_capacity(pl);
+  return PHOBIC_OK;
+}
+
 int phobic_build_u64(phobic_ctx* ctx, const uint64_t* keys, uint64_t n, const phobic_config* cfg,
-                     uint32_t* pilots_out, uint32_t* offsets_out, phobic_stage_times* times) {
+                     const phobic_outputs* out, phobic_stage_times* times) {
   if (n && !keys) return PHOBIC_E_INVALID_CONFIG;
-  return host_build(ctx, kKindU64, keys, n * 8, nullptr, 0, n, cfg, pilots_out, offsets_out, times);
+  return host_build(ctx, kKindU64, keys, n * 8, nullptr, 0, n, cfg, out, times);
 }
 
 int phobic_build_bytes(phobic_ctx* ctx, const uint8_t* bytes, const uint64_t* offsets, uint64_t n,
-                       const phobic_config* cfg, uint32_t* pilots_out, uint32_t* offsets_out,
-                       phobic_stage_times* times) {
+                       const phobic_config* cfg, const phobic_outputs* out, phobic_stage_times* times) {
   if (!offsets || (n && !bytes && offsets[n] != offsets[0])) return PHOBIC_E_INVALID_CONFIG;
   for (uint64_t i = 0; i < n; i++)
     if (offsets[i + 1] < offsets[i]) return PHOBIC_E_INVALID_CONFIG;
   // keys index the byte buffer from offsets[0]; ship the whole prefix
-  const uint64_t total = offsets[n];
-  return host_build(ctx, kKindBytes, bytes, total, offsets, (n + 1) * 8, n, cfg, pilots_out, offsets_out, times);
+  return host_build(ctx, kKindBytes, bytes, offsets[n], offsets, (n + 1) * 8, n, cfg, out, times);
 }
 
 int phobic_build_keys(phobic_ctx* ctx, const phobic_key* keys, uint64_t n, const phobic_config* cfg,
-                      uint32_t* pilots_out, uint32_t* offsets_out, phobic_stage_times* times) {
+                      const phobic_outputs* out, phobic_stage_times* times) {
   if (n && !keys) return PHOBIC_E_INVALID_CONFIG;
-  return host_build(ctx, kKindKey, keys, n * 16, nullptr, 0, n, cfg, pilots_out, offsets_out, times);
+  return host_build(ctx, kKindKey, keys, n * 16, nullptr, 0, n, cfg, out, times);
 }
 
 int phobic_build_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t n, const phobic_config* cfg,
-                            uint32_t* d_pilots, uint32_t* d_offsets, uint32_t* d_status, void* stream) {
+                            const phobic_outputs* d_out, uint32_t* d_status, void* stream) {
   KeySource src{};
   src.u64 = d_keys;
   if (n && !d_keys) return PHOBIC_E_INVALID_CONFIG;
-  return device_build(ctx, kKindU64, src, n, cfg, d_pilots, d_offsets, d_status, stream);
+  return device_build(ctx, kKindU64, src, n, cfg, d_out, d_status, stream);
 }
 
 int phobic_build_bytes_device(phobic_ctx* ctx, const uint8_t* d_bytes, const uint64_t* d_offsets, uint64_t n,
-                              const phobic_config* cfg, uint32_t* d_pilots, uint32_t* d_part_offsets,
-                              uint32_t* d_status, void* stream) {
+                              const phobic_config* cfg, const phobic_outputs* d_out, uint32_t* d_status,
+                              void* stream) {
   KeySource src{};
   src.bytes = d_bytes;
   src.offs = d_offsets;
   if (!d_offsets) return PHOBIC_E_INVALID_CONFIG;
-  return device_build(ctx, kKindBytes, src, n, cfg, d_pilots, d_part_offsets, d_status, stream);
+  return device_build(ctx, kKindBytes, src, n, cfg, d_out, d_status, stream);
 }
 
 int phobic_build_keys_device(phobic_ctx* ctx, const phobic_key* d_keys, uint64_t n, const phobic_config* cfg,
-                             uint32_t* d_pilots, uint32_t* d_offsets, uint32_t* d_status, void* stream) {
+                             const phobic_outputs* d_out, uint32_t* d_status, void* stream) {
   KeySource src{};
   src.keys = reinterpret_cast<const uint4*>(d_keys);
   if (n && !d_keys) return PHOBIC_E_INVALID_CONFIG;
-  return device_build(ctx, kKindKey, src, n, cfg, d_pilots, d_offsets, d_status, stream);
+  return device_build(ctx, kKindKey, src, n, cfg, d_out, d_status, stream);
 }
 
 int phobic_last_stage_times(phobic_ctx* ctx, phobic_stage_times* times) {
@@ -455,11 +733,12 @@ int phobic_last_stage_times(phobic_ctx* ctx, phobic_stage_times* times) {
 }
 
 int phobic_eval_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t count, const phobic_config* cfg,
-                           uint32_t np, const uint32_t* d_pilots, const uint32_t* d_offsets, uint32_t* d_out,
-                           void* stream) {
-  if (!ctx || (count && (!d_keys || !d_pilots || !d_offsets || !d_out)) || np == 0) return PHOBIC_E_INVALID_CONFIG;
+                           uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out, void* stream) {
   Plan pl;
   TRY(validate(cfg, 0, &pl));
+  if (!ctx || !d_mphf || (count && (!d_keys || !d_mphf->pilots || !d_mphf->offsets || !d_out)) || np == 0)
+    return PHOBIC_E_INVALID_CONFIG;
+  if (pl.a_ppm < 1000000u && (!d_mphf->remap || !d_mphf->remap_offsets)) return PHOBIC_E_INVALID_CONFIG;
   TRY(set_device(ctx));
   TRY(upload_fulcrums(ctx, cfg, (hipStream_t)stream));
   EvalArgs ea{};
@@ -467,9 +746,12 @@ int phobic_eval_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t cou
   ea.count = count;
   ea.seed = cfg->seed;
   ea.np = np;
+  ea.a_ppm = pl.a_ppm;
   ea.fulcs = ctx->fulcs.as<uint32_t>();
-  ea.pilots = d_pilots;
-  ea.offsets = d_offsets;
+  ea.pilots = d_mphf->pilots;
+  ea.offsets = d_mphf->offsets;
+  ea.remap = d_mphf->remap;
+  ea.remap_offsets = d_mphf->remap_offsets;
   ea.out = d_out;
   TRY_HIP(launch_eval_u64(ea, (hipStream_t)stream));
   return PHOBIC_OK;

@@ -40,6 +40,11 @@ struct HashRouteArgs {
   uint64_t* out_lo;       // records: lower1 | lower2 << 32
   uint32_t* out_meta;     // records: partition-in-bin << 16 | bucket
   uint32_t* status;       // error bit mask
+  // batched builds (sets > 0): key i of set s lies in [set_key_off[s],
+  // set_key_off[s+1]); its partition is set_part_base[s] + hi32(x * np_s)
+  uint32_t sets;
+  const uint64_t* set_key_off;
+  const uint32_t* set_part_base;
 };
 
 // Levels 2..L: split every parent bin into `fanout` child bins.
@@ -58,14 +63,29 @@ struct SplitArgs {
   uint32_t* status;
 };
 
+// Table size of a partition with ps keys at load factor a_ppm / 1e6:
+// ceil(ps * 1e6 / a_ppm); ps itself when a_ppm = 1e6 (the reference).
+__host__ __device__ inline uint32_t table_size(uint32_t ps, uint32_t a_ppm) {
+  return a_ppm >= 1000000u ? ps : (uint32_t)(((uint64_t)ps * 1000000u + a_ppm - 1) / a_ppm);
+}
+
 struct SearchArgs {
   const uint64_t* rec_lower;
   const uint32_t* rec_meta;  // bucket in the low 16 bits
   const uint32_t* pcount;
   uint64_t* sorted;  // np*cap scratch: keys of each partition in bucket order
   uint32_t np, B, cap;
+  uint32_t a_ppm;    // load factor in ppm (1e6 = alpha 1)
+  uint32_t tcap;     // table_size(cap, a_ppm)
   uint32_t* pilots;  // np*B, interleaved p + b*np
+  uint32_t* remap;   // alpha < 1: remap entries, else null
+  const uint32_t* remap_offsets;
   uint32_t* status;
+  // batched builds (sets > 0): global partition p belongs to the set s with
+  // set_part_base[s] <= p < set_part_base[s+1]; its pilots go to
+  // pilots + B*set_part_base[s], interleaved by that set's partition count
+  uint32_t sets;
+  const uint32_t* set_part_base;
 };
 
 struct EvalArgs {
@@ -73,9 +93,12 @@ struct EvalArgs {
   uint64_t count;
   uint64_t seed;
   uint32_t np;
+  uint32_t a_ppm;
   const uint32_t* fulcs;
   const uint32_t* pilots;
   const uint32_t* offsets;
+  const uint32_t* remap;
+  const uint32_t* remap_offsets;
   uint32_t* out;
 };
 
@@ -83,10 +106,16 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_hash_route(int kind, const HashRouteArgs& a, hipStream_t s);
 hipError_t launch_split_level(const SplitArgs& a, uint32_t parents, hipStream_t s);
 uint32_t route_tile_records();
-hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_t* offsets, hipStream_t s);
-size_t search_lds_bytes(uint32_t cap, uint32_t B);
+// exclusive scan of counts (a_ppm = 0) or of table_size(count) - count
+hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_t a_ppm, uint32_t* offsets,
+                                    hipStream_t s);
+size_t search_lds_bytes(uint32_t tcap, uint32_t B);
 hipError_t launch_partition_search(const SearchArgs& a, hipStream_t s);
 hipError_t launch_eval_u64(const EvalArgs& a, hipStream_t s);
+// per-set exclusive offsets from the global ones: set s's np_s + 1 entries
+// start at out + set_part_base[s] + s
+hipError_t launch_batch_offsets(const uint32_t* global_offsets, const uint32_t* set_part_base, uint32_t sets,
+                                uint32_t total_entries, uint32_t* out, hipStream_t s);
 hipError_t launch_check_bijection(const uint32_t* pos, uint64_t count, uint64_t n, uint32_t* bitmap,
                                   unsigned long long* bad, hipStream_t s);
 hipError_t launch_splitmix64(uint64_t seed, uint64_t n, uint64_t* out, hipStream_t s);

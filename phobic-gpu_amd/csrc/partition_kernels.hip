@@ -135,6 +135,17 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
     uint64_t lo[kPer];
     uint32_t meta[kPer], child[kPer];
     bool valid[kPer];
+    uint32_t set = 0;
+    if (a.sets) {  // set of this thread's first key: last s with set_key_off[s] <= i
+      const uint64_t i0 = t * kTile + threadIdx.x;
+      uint32_t lo_s = 0, hi_s = a.sets;
+      while (hi_s - lo_s > 1) {
+        const uint32_t mid = (lo_s + hi_s) / 2;
+        if (a.set_key_off[mid] <= i0) lo_s = mid;
+        else hi_s = mid;
+      }
+      set = lo_s;
+    }
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       const uint64_t i = t * kTile + k * kThreads + threadIdx.x;
@@ -144,7 +155,14 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
       child[k] = 0;
       if (valid[k]) {
         const U128 h = hash_key<KIND>(a.src, i, a.seed);
-        const uint32_t p = partition_of((uint32_t)h.lo, a.np);
+        uint32_t p;
+        if (a.sets) {
+          while (set + 1 < a.sets && a.set_key_off[set + 1] <= i) set++;
+          const uint32_t pb = a.set_part_base[set];
+          p = pb + partition_of((uint32_t)h.lo, a.set_part_base[set + 1] - pb);
+        } else {
+          p = partition_of((uint32_t)h.lo, a.np);
+        }
         const uint32_t b = bucket_of((uint32_t)(h.lo >> 32), fl);
         const uint32_t c = p / a.G;
         lo[k] = h.hi;

@@ -34,12 +34,10 @@ namespace {
 constexpr uint32_t kWavesPerGroup = 4;
 constexpr uint32_t kMaxBucket = 256;  // sort_sum.comp BINS, search.comp MAX_BUCKET_SIZE
 
-struct Lds {
-  uint32_t E_words, coll_words, B;
-  __device__ static uint32_t words(uint32_t cap, uint32_t B) {
-    return ((2 * cap + 128) / 32 + 4) + ((cap + 31) / 32 + 1) + 5 * B + 256;
-  }
-};
+// LDS words of one partition for tables of at most tcap slots
+__host__ __device__ inline uint32_t lds_words(uint32_t tcap, uint32_t B) {
+  return ((2 * tcap + 128) / 32 + 4) + ((tcap + 31) / 32 + 1) + 5 * B + 256;
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const uint32_t l = __lane_id();
@@ -109,9 +107,9 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
   extern __shared__ __align__(16) uint32_t smem[];
   const uint32_t lane = __lane_id();
   const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t cap = a.cap, B = a.B;
-  const uint32_t EW = (2 * cap + 128) / 32 + 4, CW = (cap + 31) / 32 + 1;
-  uint32_t* E = smem + wave * Lds::words(cap, B);
+  const uint32_t cap = a.cap, B = a.B, tcap = a.tcap;
+  const uint32_t EW = (2 * tcap + 128) / 32 + 4, CW = (tcap + 31) / 32 + 1;
+  uint32_t* E = smem + wave * lds_words(tcap, B);
   uint32_t* coll = E + EW;
   uint32_t* bcnt = coll + CW;   // key count per bucket
   uint32_t* bcur = bcnt + B;    // placement cursor
@@ -124,8 +122,25 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
   if (p >= a.np) return;
   const uint32_t ps = a.pcount[p];
   const uint64_t base = (uint64_t)p * cap;
+  // output: pilots[p + b*np] (one build) or the owning set's slice
+  uint32_t* out_pilots = a.pilots;
+  uint32_t out_p = p, out_np = a.np;
+  if (a.sets) {
+    uint32_t lo_s = 0, hi_s = a.sets;
+    while (hi_s - lo_s > 1) {
+      const uint32_t mid = (lo_s + hi_s) / 2;
+      if (a.set_part_base[mid] <= p) lo_s = mid;
+      else hi_s = mid;
+    }
+    // skip empty sets that share the same base
+    while (lo_s + 1 < a.sets && a.set_part_base[lo_s + 1] <= p) lo_s++;
+    const uint32_t pb = a.set_part_base[lo_s];
+    out_pilots = a.pilots + (uint64_t)B * pb;
+    out_p = p - pb;
+    out_np = a.set_part_base[lo_s + 1] - pb;
+  }
   if (ps > cap) {  // the scatter stage already raised kErrPartitionOverflow
-    for (uint32_t b = lane; b < B; b += 64) a.pilots[p + (uint64_t)b * a.np] = 0;
+    for (uint32_t b = lane; b < B; b += 64) out_pilots[out_p + (uint64_t)b * out_np] = 0;
     return;
   }
   for (uint32_t i = lane; i < EW; i += 64) E[i] = 0;
@@ -149,7 +164,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
   }
   if (__any(over)) {
     if (lane == 0) atomicOr(a.status, 1u << kErrBucketOverflow);
-    for (uint32_t b = lane; b < B; b += 64) a.pilots[p + (uint64_t)b * a.np] = 0;
+    for (uint32_t b = lane; b < B; b += 64) out_pilots[out_p + (uint64_t)b * out_np] = 0;
     return;
   }
   wave_sync();
@@ -222,11 +237,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
   }
   wave_sync();
 
-  // 5. pilot search
-  const uint64_t M = ~0ull / (ps ? ps : 1) + 1;
-  const uint32_t L = 2 * ps + 128;  // valid bits of E
-  const uint32_t nblk = (ps + 63) / 64;
-  const uint64_t blk_mask = lane < nblk ? low_mask(ps - 64 * lane) : 0ull;
+  // 5. pilot search over a table of tm slots (tm = ps unless alpha < 1)
+  const uint32_t tm = table_size(ps, a.a_ppm);
+  const uint64_t M = ~0ull / (tm ? tm : 1) + 1;
+  const uint32_t L = 2 * tm + 128;  // valid bits of E
+  const uint32_t nblk = (tm + 63) / 64;
+  const uint64_t blk_mask = lane < nblk ? low_mask(tm - 64 * lane) : 0ull;
   uint32_t err = 0;
 
   uint64_t cur[4];
@@ -260,16 +276,16 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
     uint32_t pos[4] = {0, 0, 0, 0};
     uint32_t pilot = 0, dsel = 0;
     for (;;) {
-      const uint64_t vbase = (uint64_t)ps * pilot;
+      const uint64_t vbase = (uint64_t)tm * pilot;
       if (vbase > 0x7FFFFFFFull) {
         err = kErrPilotOverflow;
         break;
       }
-      const uint32_t dlimit = (uint32_t)min((uint64_t)ps, 0x80000000ull - vbase);  // d < dlimit
+      const uint32_t dlimit = (uint32_t)min((uint64_t)tm, 0x80000000ull - vbase);  // d < dlimit
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const uint32_t k = lane + 64 * j;
-        if (k < s) pos[j] = fastmod(mix32((uint32_t)cur[j], mix32((uint32_t)(cur[j] >> 32), pilot)) >> 1, M, ps);
+        if (k < s) pos[j] = fastmod(mix32((uint32_t)cur[j], mix32((uint32_t)(cur[j] >> 32), pilot)) >> 1, M, tm);
       }
       if (s > 1) {
         bool c = false;
@@ -347,11 +363,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
       const uint32_t k = lane + 64 * j;
       if (k < s) {
         uint32_t q = pos[j] + dsel;
-        if (q >= ps) q -= ps;
-        for (uint32_t x = q; x < L; x += ps) atomicOr(&E[x >> 5], 1u << (x & 31));
+        if (q >= tm) q -= tm;
+        for (uint32_t x = q; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
       }
     }
-    if (lane == 0) pil[cb] = dsel + ps * pilot;
+    if (lane == 0) pil[cb] = dsel + tm * pilot;
     wave_sync();
 #pragma unroll
     for (int j = 0; j < 4; j++) cur[j] = nxt[j];
@@ -361,18 +377,59 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
   }
   if (err && lane == 0) atomicOr(a.status, 1u << err);
   wave_sync();
-  for (uint32_t b = lane; b < B; b += 64) a.pilots[p + (uint64_t)b * a.np] = pil[b];
+
+  // 6. alpha < 1: the k-th used slot of [ps, tm) stands for the k-th free
+  // slot of [0, ps).  R[0..K) first receives the free slots in order, then
+  // the high slots are filled in descending order (each reads R[rank] with
+  // rank <= its own index before anything at or below it is overwritten).
+  if (a.remap && !err && tm > ps) {
+    uint32_t* R = a.remap + a.remap_offsets[p];
+    const uint32_t W0 = (ps + 31) / 32, wpl = (W0 + 63) / 64;
+    const uint32_t w_lo = min(W0, lane * wpl), w_hi = min(W0, (lane + 1) * wpl);
+    auto free_word = [&](uint32_t w) {
+      uint32_t fr = ~E[w];
+      if (w == W0 - 1 && (ps & 31)) fr &= (1u << (ps & 31)) - 1;
+      return fr;
+    };
+    uint32_t cnt = 0;
+    for (uint32_t w = w_lo; w < w_hi; w++) cnt += __popc(free_word(w));
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t K = __builtin_amdgcn_readlane(incl, 63);
+    uint32_t rank = incl - cnt;
+    for (uint32_t w = w_lo; w < w_hi; w++) {
+      uint32_t fr = free_word(w);
+      while (fr) {
+        const uint32_t bit = (uint32_t)__builtin_ctz(fr);
+        fr &= fr - 1;
+        R[rank++] = 32 * w + bit;
+      }
+    }
+    wave_sync();
+    const uint32_t H = tm - ps;
+    uint32_t above = 0;
+    for (int c = (int)((H + 63) / 64) - 1; c >= 0; c--) {
+      const uint32_t j = 64 * (uint32_t)c + lane;
+      const uint32_t q = ps + j;
+      const bool used = j < H && ((E[q >> 5] >> (q & 31)) & 1u);
+      const uint64_t bal = __ballot(used);
+      const uint32_t cnt_c = __popcll(bal);
+      const uint32_t r = K - above - cnt_c + __popcll(bal & lanemask_lt());
+      const uint32_t src = used ? R[r] : 0u;
+      wave_sync();
+      if (j < H) R[j] = src;
+      wave_sync();
+      above += cnt_c;
+    }
+  }
+  for (uint32_t b = lane; b < B; b += 64) out_pilots[out_p + (uint64_t)b * out_np] = pil[b];
 }
 
-size_t search_lds_bytes(uint32_t cap, uint32_t B) {
-  const size_t words = ((2 * (size_t)cap + 128) / 32 + 4) + ((cap + 31) / 32 + 1) + 5 * (size_t)B + 256;
-  return words * 4 * kWavesPerGroup;
-}
+size_t search_lds_bytes(uint32_t tcap, uint32_t B) { return (size_t)lds_words(tcap, B) * 4 * kWavesPerGroup; }
 
 hipError_t launch_partition_search(const SearchArgs& a, hipStream_t s) {
   if (a.np == 0) return hipSuccess;
   const uint32_t groups = (a.np + kWavesPerGroup - 1) / kWavesPerGroup;
-  hipLaunchKernelGGL(k_partition_search, dim3(groups), dim3(64 * kWavesPerGroup), search_lds_bytes(a.cap, a.B), s, a);
+  hipLaunchKernelGGL(k_partition_search, dim3(groups), dim3(64 * kWavesPerGroup), search_lds_bytes(a.tcap, a.B), s, a);
   return hipGetLastError();
 }
 

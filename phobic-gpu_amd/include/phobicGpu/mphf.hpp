@@ -9,6 +9,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "../encoders/encoders.hpp"
 #include "hasher.hpp"
 #include "mphf_config.h"
 
@@ -42,6 +43,10 @@ class MPHF {
   uint64_t keys = 0;
   PilotEncoder pilots;
   PartitionOffsetEncoder partitionOffsets;
+  // alpha < 1 (extension): table sizes and slot remapping
+  MPHFconfig cfg;
+  compact remap, remapOffsets;
+  bool hasRemap = false;
 
   // hashing.glsl:11-17 / mphf.hpp:24-30
   static inline uint32_t hash(uint32_t pilot, uint32_t key) {
@@ -51,10 +56,10 @@ class MPHF {
     return (key >> 16) ^ key;
   }
 
-  static inline uint32_t hashPos(uint32_t pilot, uint32_t lower1, uint32_t lower2, uint32_t ps) {
-    const uint32_t hp = pilot / ps;
+  static inline uint32_t hashPos(uint32_t pilot, uint32_t lower1, uint32_t lower2, uint32_t tableSize) {
+    const uint32_t hp = pilot / tableSize;
     const uint32_t hv = hash(lower1, hash(lower2, hp)) >> 1;
-    return (hv + pilot) % ps;
+    return (hv + pilot) % tableSize;
   }
 
   // mphf.hpp:42-49
@@ -83,6 +88,7 @@ class MPHF {
     partitions = parts;
     partitionSize = config.partitionSize;
     seed = config.seed;
+    cfg = config;
     keys = offsetsIn.empty() ? 0 : offsetsIn.back();
     partitionOffsets.encode(offsetsIn.begin(), config.partitionSize, parts + 1);
     if (parts) pilots.encode(pilotsIn.begin(), parts, config.bucketCountPerPartition);
@@ -96,11 +102,24 @@ class MPHF {
     const uint32_t pilot = (uint32_t)pilots.access(partition, bucket);
     const uint32_t off = (uint32_t)partitionOffsets.access(partition);
     const uint32_t ps = (uint32_t)partitionOffsets.access(partition + 1) - off;
-    return off + hashPos(pilot, key.lower1, key.lower2, ps);
+    if (!hasRemap) return off + hashPos(pilot, key.lower1, key.lower2, ps);
+    uint32_t pos = hashPos(pilot, key.lower1, key.lower2, cfg.tableSize(ps));
+    if (pos >= ps) pos = (uint32_t)remap.access(remapOffsets.access(partition) + pos - ps);
+    return off + pos;
+  }
+
+  // alpha < 1 (extension): slots [ps, m) of partition p stand for
+  // remap[remapOffsets[p] + q - ps]
+  void setRemap(const std::vector<uint32_t>& entries, const std::vector<uint32_t>& offsets) {
+    remap.encode(entries.begin(), entries.size());
+    remapOffsets.encode(offsets.begin(), offsets.size());
+    hasRemap = true;
   }
 
   float getBitsPerKey() const {
-    return float(pilots.num_bits() + partitionOffsets.num_bits() + 32 * fulcs.size() + 32 * uint64_t(partitions)) /
+    const uint64_t rbits = hasRemap ? remap.num_bits() + remapOffsets.num_bits() : 0;
+    return float(pilots.num_bits() + partitionOffsets.num_bits() + rbits + 32 * fulcs.size() +
+                 32 * uint64_t(partitions)) /
            float(keys);
   }
 
@@ -127,6 +146,11 @@ class MPHF {
     for (size_t i = 0; i + 1 < fulcs.size(); i += 2) s.put(uint64_t(fulcs[i]) | (uint64_t(fulcs[i + 1]) << 32));
     if (partitions) pilots.visit(s);
     partitionOffsets.visit(s);
+    if (hasRemap) {  // extension section, present only for alpha < 1
+      s.put(cfg.alphaPpm());
+      remapOffsets.visit(s);
+      remap.visit(s);
+    }
     return out;
   }
 };

@@ -34,6 +34,7 @@ class MPHFbuilder {
     c.lambda = config.m_averageBucketSize;
     c.partition_size = config.partitionSize;
     c.seed = config.seed;
+    c.alpha = config.alpha;
     if (config.hasCustomBucketer()) {
       c.buckets = config.bucketCountPerPartition;
       c.fulcrums = config.getFulcs().data();
@@ -61,14 +62,18 @@ class MPHFbuilder {
     uint32_t np = 0, B = 0;
     check(phobic_query_sizes(&cfg, n, &np, &B), "phobic_query_sizes");
     std::vector<uint32_t> pilots((size_t)np * B), offsets((size_t)np + 1);
+    uint64_t rcap = 0;
+    check(phobic_remap_capacity(&cfg, n, &rcap), "phobic_remap_capacity");
+    std::vector<uint32_t> remap(rcap ? rcap : 1), remapOffsets(rcap ? (size_t)np + 1 : 1);
+    phobic_outputs out{pilots.data(), offsets.data(), rcap ? remap.data() : nullptr,
+                       rcap ? remapOffsets.data() : nullptr};
     phobic_stage_times t{};
     int st;
     if constexpr (Mphf::noHash()) {
       static_assert(std::is_same_v<keyType, Key>, "nohash MPHFs are built from Key values");
-      st = phobic_build_keys(ctx, reinterpret_cast<const phobic_key*>(keys.data()), n, &cfg, pilots.data(),
-                             offsets.data(), &t);
+      st = phobic_build_keys(ctx, reinterpret_cast<const phobic_key*>(keys.data()), n, &cfg, &out, &t);
     } else if constexpr (std::is_same_v<keyType, uint64_t>) {
-      st = phobic_build_u64(ctx, keys.data(), n, &cfg, pilots.data(), offsets.data(), &t);
+      st = phobic_build_u64(ctx, keys.data(), n, &cfg, &out, &t);
     } else if constexpr (std::is_same_v<keyType, std::string>) {
       std::vector<uint64_t> offs(n + 1, 0);
       for (uint64_t i = 0; i < n; i++) offs[i + 1] = offs[i] + keys[i].size();
@@ -76,7 +81,7 @@ class MPHFbuilder {
       for (uint64_t i = 0; i < n; i++)
         if (!keys[i].empty()) std::memcpy(bytes.data() + offs[i], keys[i].data(), keys[i].size());
       timer.addLabel("input_packing");
-      st = phobic_build_bytes(ctx, bytes.data(), offs.data(), n, &cfg, pilots.data(), offsets.data(), &t);
+      st = phobic_build_bytes(ctx, bytes.data(), offs.data(), n, &cfg, &out, &t);
     } else if constexpr (std::is_same_v<keyType, std::pair<uint64_t, uint64_t>>) {
       std::vector<uint64_t> offs(n + 1), words(2 * n);
       for (uint64_t i = 0; i < n; i++) {
@@ -85,8 +90,7 @@ class MPHFbuilder {
         words[2 * i + 1] = keys[i].second;
       }
       offs[n] = 16 * n;
-      st = phobic_build_bytes(ctx, reinterpret_cast<const uint8_t*>(words.data()), offs.data(), n, &cfg,
-                              pilots.data(), offsets.data(), &t);
+      st = phobic_build_bytes(ctx, reinterpret_cast<const uint8_t*>(words.data()), offs.data(), n, &cfg, &out, &t);
     } else {
       static_assert(sizeof(keyType) == 0, "supported key types: uint64_t, std::string, std::pair<u64,u64>, Key");
     }
@@ -98,6 +102,10 @@ class MPHFbuilder {
       timer.addLabelManually(std::string("GPU_") + t.names[i], acc);
     }
     timer.addLabel("result_transfer");
+    if (rcap) {
+      remap.resize(remapOffsets[np]);
+      f.setRemap(remap, remapOffsets);
+    }
     f.setData(pilots, offsets, np, config);
     timer.addLabel("encoding");
     return timer;

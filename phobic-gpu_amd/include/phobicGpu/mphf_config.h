@@ -22,7 +22,15 @@ struct MPHFconfig {
   uint32_t sortingBins;
   uint32_t bucketCountPerPartition;
   double m_averageBucketSize;
-  uint64_t seed = 0;  // extension: XXH3 seed (the reference hashes with seed 0)
+  uint64_t seed = 0;   // extension: XXH3 seed (the reference hashes with seed 0)
+  double alpha = 1.0;  // extension: load factor in [0.5, 1]; 1 = the reference
+
+  // table size of a partition with ps keys: ceil(ps / alpha) in ppm arithmetic
+  uint32_t tableSize(uint32_t ps) const {
+    const uint32_t a = alphaPpm();
+    return a >= 1000000u ? ps : (uint32_t)(((uint64_t)ps * 1000000u + a - 1) / a);
+  }
+  uint32_t alphaPpm() const { return (uint32_t)std::llround((alpha == 0.0 ? 1.0 : alpha) * 1e6); }
 
   // B = ceil(P / lambda) with the OptBucketer table (mphf_config.h:26-32)
   MPHFconfig(double averageBucketSize = 8.0, uint32_t partitionSize = 2048)

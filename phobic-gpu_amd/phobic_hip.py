@@ -46,6 +46,17 @@ class Config(ctypes.Structure):
         ("buckets", ctypes.c_uint32),
         ("seed", ctypes.c_uint64),
         ("fulcrums", ctypes.c_void_p),
+        ("alpha", ctypes.c_double),
+    ]
+
+
+class Outputs(ctypes.Structure):
+    """phobic_outputs: host or device addresses of the build outputs."""
+    _fields_ = [
+        ("pilots", ctypes.c_void_p),
+        ("offsets", ctypes.c_void_p),
+        ("remap", ctypes.c_void_p),
+        ("remap_offsets", ctypes.c_void_p),
     ]
 
 
@@ -69,6 +80,7 @@ EXPORTS = [
     "phobic_ctx_destroy",
     "phobic_status_string",
     "phobic_query_sizes",
+    "phobic_remap_capacity",
     "phobic_fulcrums",
     "phobic_xxh3_128",
     "phobic_build_u64",
@@ -77,6 +89,20 @@ EXPORTS = [
     "phobic_build_u64_device",
     "phobic_build_bytes_device",
     "phobic_build_keys_device",
+    "phobic_batch_create",
+    "phobic_batch_layout",
+    "phobic_batch_launch",
+    "phobic_batch_graph_launch",
+    "phobic_batch_destroy",
+    "phobic_server_create",
+    "phobic_server_submit",
+    "phobic_server_poll",
+    "phobic_server_wait",
+    "phobic_server_result",
+    "phobic_server_fetch",
+    "phobic_server_release",
+    "phobic_server_stats_get",
+    "phobic_server_destroy",
     "phobic_last_stage_times",
     "phobic_eval_u64_device",
     "phobic_check_bijection_device",
@@ -97,21 +123,37 @@ def load() -> ctypes.CDLL:
     P, U32, U64, I = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
     cfgp = ctypes.POINTER(Config)
     tp = ctypes.POINTER(StageTimes)
+    op = ctypes.POINTER(Outputs)
     sig = {
         "phobic_ctx_create": (I, [I, ctypes.POINTER(P)]),
         "phobic_ctx_destroy": (I, [P]),
         "phobic_status_string": (ctypes.c_char_p, [I]),
         "phobic_query_sizes": (I, [cfgp, U64, ctypes.POINTER(U32), ctypes.POINTER(U32)]),
+        "phobic_remap_capacity": (I, [cfgp, U64, ctypes.POINTER(U64)]),
         "phobic_fulcrums": (I, [cfgp, P]),
         "phobic_xxh3_128": (I, [P, U64, U64, ctypes.POINTER(U64), ctypes.POINTER(U64)]),
-        "phobic_build_u64": (I, [P, P, U64, cfgp, P, P, tp]),
-        "phobic_build_bytes": (I, [P, P, P, U64, cfgp, P, P, tp]),
-        "phobic_build_keys": (I, [P, P, U64, cfgp, P, P, tp]),
-        "phobic_build_u64_device": (I, [P, P, U64, cfgp, P, P, P, P]),
-        "phobic_build_bytes_device": (I, [P, P, P, U64, cfgp, P, P, P, P]),
-        "phobic_build_keys_device": (I, [P, P, U64, cfgp, P, P, P, P]),
+        "phobic_build_u64": (I, [P, P, U64, cfgp, op, tp]),
+        "phobic_build_bytes": (I, [P, P, P, U64, cfgp, op, tp]),
+        "phobic_build_keys": (I, [P, P, U64, cfgp, op, tp]),
+        "phobic_build_u64_device": (I, [P, P, U64, cfgp, op, P, P]),
+        "phobic_build_bytes_device": (I, [P, P, P, U64, cfgp, op, P, P]),
+        "phobic_build_keys_device": (I, [P, P, U64, cfgp, op, P, P]),
+        "phobic_batch_create": (I, [P, cfgp, P, U32, ctypes.POINTER(P)]),
+        "phobic_batch_layout": (I, [P, P, ctypes.POINTER(U32)]),
+        "phobic_batch_launch": (I, [P, P, P, P, P, P]),
+        "phobic_batch_graph_launch": (I, [P, P, P, P, P, P]),
+        "phobic_batch_destroy": (I, [P]),
+        "phobic_server_create": (I, [P, ctypes.POINTER(P)]),
+        "phobic_server_submit": (I, [P, P, ctypes.POINTER(U64)]),
+        "phobic_server_poll": (I, [P, U64, ctypes.POINTER(I)]),
+        "phobic_server_wait": (I, [P, U64, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_double)]),
+        "phobic_server_result": (I, [P, U64, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U64), op]),
+        "phobic_server_fetch": (I, [P, U64, op]),
+        "phobic_server_release": (I, [P, U64]),
+        "phobic_server_stats_get": (I, [P, P]),
+        "phobic_server_destroy": (I, [P]),
         "phobic_last_stage_times": (I, [P, tp]),
-        "phobic_eval_u64_device": (I, [P, P, U64, cfgp, U32, P, P, P, P]),
+        "phobic_eval_u64_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
         "phobic_check_bijection_device": (I, [P, P, U64, U64, P, P, P]),
         "phobic_gen_splitmix64_device": (I, [U64, U64, P, P]),
     }
@@ -123,9 +165,18 @@ def load() -> ctypes.CDLL:
     return lib
 
 
-def make_config(lam: float = 8.0, partition_size: int = 2048, seed: int = 0) -> Config:
+def make_config(lam: float = 8.0, partition_size: int = 2048, seed: int = 0, alpha: float = 1.0) -> Config:
     """MPHFconfig(averageBucketSize, partitionSize) defaults (mphf_config.h:26)."""
-    return Config(float(lam), int(partition_size), 0, int(seed), None)
+    return Config(float(lam), int(partition_size), 0, int(seed), None, float(alpha))
+
+
+def remap_capacity(cfg: Config, n: int) -> int:
+    lib = load()
+    v = ctypes.c_uint64()
+    st = lib.phobic_remap_capacity(ctypes.byref(cfg), n, ctypes.byref(v))
+    if st:
+        raise PhobicError(st, "remap_capacity")
+    return v.value
 
 
 def sizes(cfg: Config, n: int) -> tuple[int, int]:
@@ -168,6 +219,13 @@ class BuildResult:
     np_: int
     B: int
     times: dict = field(default_factory=dict)
+    remap: np.ndarray | None = None          # alpha < 1
+    remap_offsets: np.ndarray | None = None  # alpha < 1
+
+
+def device_outputs(pilots, offsets, remap=None, remap_offsets=None) -> Outputs:
+    """phobic_outputs from device buffers (hip_rt.DeviceBuffer or addresses)."""
+    return Outputs(_ptr(pilots), _ptr(offsets), _ptr(remap) or None, _ptr(remap_offsets) or None)
 
 
 class Context:
@@ -201,13 +259,21 @@ class Context:
     # ---- host-buffer builds ----
     def _host(self, fn, args, cfg: Config, n: int) -> BuildResult:
         np_, B = sizes(cfg, n)
-        pilots = np.zeros(np_ * B, dtype=np.uint32)
+        pilots = np.zeros(max(np_ * B, 1), dtype=np.uint32)
         offsets = np.zeros(np_ + 1, dtype=np.uint32)
+        rcap = remap_capacity(cfg, n)
+        remap = np.zeros(max(rcap, 1), dtype=np.uint32) if rcap else None
+        roff = np.zeros(np_ + 1, dtype=np.uint32) if rcap else None
+        out = Outputs(pilots.ctypes.data, offsets.ctypes.data, remap.ctypes.data if rcap else None,
+                      roff.ctypes.data if rcap else None)
         t = StageTimes()
-        st = fn(self.h, *args, ctypes.byref(cfg), pilots.ctypes.data, offsets.ctypes.data, ctypes.byref(t))
+        st = fn(self.h, *args, ctypes.byref(cfg), ctypes.byref(out), ctypes.byref(t))
         if st:
             raise PhobicError(st, fn.__name__)
-        return BuildResult(pilots, offsets, np_, B, t.as_dict())
+        r = BuildResult(pilots[:np_ * B], offsets, np_, B, t.as_dict())
+        if rcap:
+            r.remap, r.remap_offsets = remap[:int(roff[-1])], roff
+        return r
 
     def build_u64(self, keys: np.ndarray, cfg: Config) -> BuildResult:
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
@@ -225,24 +291,21 @@ class Context:
         return self._host(self.lib.phobic_build_bytes, (buf.ctypes.data, offsets.ctypes.data, n), cfg, n)
 
     # ---- device builds (device buffers; enqueue on `stream`, default null) ----
-    def build_u64_device(self, d_keys, n, cfg: Config, d_pilots, d_offsets, d_status, stream=None) -> None:
-        st = self.lib.phobic_build_u64_device(
-            self.h, _ptr(d_keys), n, ctypes.byref(cfg), _ptr(d_pilots), _ptr(d_offsets),
-            _ptr(d_status), _stream(stream))
+    def build_u64_device(self, d_keys, n, cfg: Config, out: Outputs, d_status, stream=None) -> None:
+        st = self.lib.phobic_build_u64_device(self.h, _ptr(d_keys), n, ctypes.byref(cfg), ctypes.byref(out),
+                                              _ptr(d_status), _stream(stream))
         if st:
             raise PhobicError(st, "build_u64_device")
 
-    def build_bytes_device(self, d_bytes, d_offs, n, cfg, d_pilots, d_offsets, d_status, stream=None) -> None:
-        st = self.lib.phobic_build_bytes_device(
-            self.h, _ptr(d_bytes), _ptr(d_offs), n, ctypes.byref(cfg), _ptr(d_pilots), _ptr(d_offsets),
-            _ptr(d_status), _stream(stream))
+    def build_bytes_device(self, d_bytes, d_offs, n, cfg, out: Outputs, d_status, stream=None) -> None:
+        st = self.lib.phobic_build_bytes_device(self.h, _ptr(d_bytes), _ptr(d_offs), n, ctypes.byref(cfg),
+                                                ctypes.byref(out), _ptr(d_status), _stream(stream))
         if st:
             raise PhobicError(st, "build_bytes_device")
 
-    def build_keys_device(self, d_keys, n, cfg, d_pilots, d_offsets, d_status, stream=None) -> None:
-        st = self.lib.phobic_build_keys_device(
-            self.h, _ptr(d_keys), n, ctypes.byref(cfg), _ptr(d_pilots), _ptr(d_offsets), _ptr(d_status),
-            _stream(stream))
+    def build_keys_device(self, d_keys, n, cfg, out: Outputs, d_status, stream=None) -> None:
+        st = self.lib.phobic_build_keys_device(self.h, _ptr(d_keys), n, ctypes.byref(cfg), ctypes.byref(out),
+                                               _ptr(d_status), _stream(stream))
         if st:
             raise PhobicError(st, "build_keys_device")
 
@@ -253,10 +316,9 @@ class Context:
             raise PhobicError(st, "last_stage_times")
         return t.as_dict()
 
-    def eval_u64_device(self, d_keys, count, cfg, np_, d_pilots, d_offsets, d_out, stream=None) -> None:
-        st = self.lib.phobic_eval_u64_device(
-            self.h, _ptr(d_keys), count, ctypes.byref(cfg), np_, _ptr(d_pilots), _ptr(d_offsets),
-            _ptr(d_out), _stream(stream))
+    def eval_u64_device(self, d_keys, count, cfg, np_, mphf: Outputs, d_out, stream=None) -> None:
+        st = self.lib.phobic_eval_u64_device(self.h, _ptr(d_keys), count, ctypes.byref(cfg), np_,
+                                             ctypes.byref(mphf), _ptr(d_out), _stream(stream))
         if st:
             raise PhobicError(st, "eval_u64_device")
 
@@ -265,6 +327,193 @@ class Context:
             self.h, _ptr(d_pos), count, n, _ptr(d_bitmap), _ptr(d_bad), _stream(stream))
         if st:
             raise PhobicError(st, "check_bijection_device")
+
+
+KEYS_U64, KEYS_BYTES = 0, 1
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("keys_on_device", ctypes.c_int),
+        ("keys", ctypes.c_void_p),
+        ("offsets", ctypes.c_void_p),
+        ("bytes_len", ctypes.c_uint64),
+        ("n", ctypes.c_uint64),
+        ("cfg", Config),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("device", ctypes.c_int),
+        ("hbm_budget", ctypes.c_uint64),
+        ("queue_capacity", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
+        ("keys_built", ctypes.c_uint64),
+        ("hbm_in_use", ctypes.c_uint64),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("max_ms", ctypes.c_double),
+        ("span_s", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Server:
+    """phobic_server: continuous-batching builds with device-resident results.
+    Host arrays passed to submit() are kept referenced until the request is
+    released."""
+
+    def __init__(self, slots: int = 4, hbm_budget: int = 200 << 30, device: int = 0, queue_capacity: int = 0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        cfg = ServerConfig(slots, device, hbm_budget, queue_capacity)
+        st = self.lib.phobic_server_create(ctypes.byref(cfg), ctypes.byref(h))
+        if st:
+            raise PhobicError(st, "server_create")
+        self.h = h
+        self._hold: dict[int, tuple] = {}
+        self._cfgs: dict[int, Config] = {}
+
+    def submit_u64(self, keys, cfg: Config, n: int | None = None) -> int:
+        """keys: numpy u64 array (host) or a device buffer with n given."""
+        on_dev = not isinstance(keys, np.ndarray)
+        if not on_dev:
+            keys = np.ascontiguousarray(keys, dtype=np.uint64)
+            n = keys.size
+        req = Request(KEYS_U64, int(on_dev), _ptr(keys) or None, None, 0, n, cfg)
+        return self._submit(req, (keys,), cfg)
+
+    def submit_bytes(self, data: np.ndarray, offsets: np.ndarray, cfg: Config) -> int:
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        buf = data if data.size else np.zeros(1, np.uint8)
+        n = offsets.size - 1
+        req = Request(KEYS_BYTES, 0, buf.ctypes.data, offsets.ctypes.data, int(offsets[-1]), n, cfg)
+        return self._submit(req, (buf, offsets), cfg)
+
+    def _submit(self, req: Request, hold: tuple, cfg: Config) -> int:
+        rid = ctypes.c_uint64()
+        st = self.lib.phobic_server_submit(self.h, ctypes.byref(req), ctypes.byref(rid))
+        if st:
+            raise PhobicError(st, "server_submit")
+        self._hold[rid.value] = hold
+        self._cfgs[rid.value] = cfg
+        return rid.value
+
+    def wait(self, rid: int) -> tuple[int, float]:
+        st, lat = ctypes.c_int(), ctypes.c_double()
+        e = self.lib.phobic_server_wait(self.h, rid, ctypes.byref(st), ctypes.byref(lat))
+        if e:
+            raise PhobicError(e, "server_wait")
+        return st.value, lat.value
+
+    def fetch(self, rid: int) -> BuildResult:
+        np_, B, used = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        dev = Outputs()
+        st = self.lib.phobic_server_result(self.h, rid, ctypes.byref(np_), ctypes.byref(B), ctypes.byref(used),
+                                           ctypes.byref(dev))
+        if st:
+            raise PhobicError(st, "server_result")
+        pil = np.zeros(max(np_.value * B.value, 1), np.uint32)
+        off = np.zeros(np_.value + 1, np.uint32)
+        has_remap = bool(dev.remap)
+        rm = np.zeros(max(used.value, 1), np.uint32) if has_remap else None
+        ro = np.zeros(np_.value + 1, np.uint32) if has_remap else None
+        out = Outputs(pil.ctypes.data, off.ctypes.data, rm.ctypes.data if has_remap else None,
+                      ro.ctypes.data if has_remap else None)
+        st = self.lib.phobic_server_fetch(self.h, rid, ctypes.byref(out))
+        if st:
+            raise PhobicError(st, "server_fetch")
+        r = BuildResult(pil[:np_.value * B.value], off, np_.value, B.value)
+        if has_remap:
+            r.remap, r.remap_offsets = rm[:used.value], ro
+        return r
+
+    def release(self, rid: int) -> None:
+        st = self.lib.phobic_server_release(self.h, rid)
+        if st:
+            raise PhobicError(st, "server_release")
+        self._hold.pop(rid, None)
+        self._cfgs.pop(rid, None)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        self.lib.phobic_server_stats_get(self.h, ctypes.byref(s))
+        return s.as_dict()
+
+    def close(self):
+        if self.h:
+            self.lib.phobic_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """phobic_batch: many independent u64 key sets built as one device build."""
+
+    def __init__(self, ctx: "Context", cfg: Config, set_offsets: np.ndarray):
+        self.lib = load()
+        self.cfg = cfg
+        self.set_offsets = np.ascontiguousarray(set_offsets, dtype=np.uint64)
+        self.sets = self.set_offsets.size - 1
+        h = ctypes.c_void_p()
+        st = self.lib.phobic_batch_create(ctx.h, ctypes.byref(cfg), self.set_offsets.ctypes.data, self.sets,
+                                          ctypes.byref(h))
+        if st:
+            raise PhobicError(st, "batch_create")
+        self.h = h
+        self.part_base = np.zeros(self.sets + 1, dtype=np.uint32)
+        b = ctypes.c_uint32()
+        self.lib.phobic_batch_layout(self.h, self.part_base.ctypes.data, ctypes.byref(b))
+        self.B = b.value
+
+    @property
+    def total_pilots(self) -> int:
+        return int(self.part_base[-1]) * self.B
+
+    @property
+    def total_offsets(self) -> int:
+        return int(self.part_base[-1]) + self.sets
+
+    def launch(self, d_keys, d_pilots, d_offsets, d_status, stream=None, graph: bool = False) -> None:
+        fn = self.lib.phobic_batch_graph_launch if graph else self.lib.phobic_batch_launch
+        st = fn(self.h, _ptr(d_keys), _ptr(d_pilots), _ptr(d_offsets), _ptr(d_status), _stream(stream))
+        if st:
+            raise PhobicError(st, fn.__name__)
+
+    def split(self, pilots: np.ndarray, offsets: np.ndarray, s: int) -> tuple[np.ndarray, np.ndarray]:
+        """Set s's pilots and offsets out of the batch outputs."""
+        pb, pe = int(self.part_base[s]), int(self.part_base[s + 1])
+        return pilots[self.B * pb:self.B * pe], offsets[pb + s:pe + s + 1]
+
+    def close(self):
+        if self.h:
+            self.lib.phobic_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gen_splitmix64_device(seed: int, n: int, d_keys, stream=None) -> None:

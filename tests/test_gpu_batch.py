@@ -1,0 +1,82 @@
+"""Batched builds: every set's pilots and offsets equal building it alone
+(and the oracle), eagerly and through replays of the captured hipGraph."""
+import numpy as np
+import pytest
+
+import hip_rt
+import phobic_hip as ph
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def make_sets(count, lo, hi, seed, extra=()):
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+    sizes = np.concatenate([sizes, np.array(extra, dtype=np.int64)])
+    offs = np.zeros(sizes.size + 1, np.uint64)
+    offs[1:] = np.cumsum(sizes)
+    keys = np.concatenate([po.splitmix64(1000 + s, int(n)) for s, n in enumerate(sizes)] or [np.zeros(0, np.uint64)])
+    return sizes, offs, keys
+
+
+def run_batch(ctx, cfg, offs, keys, graph=False, reps=1, between=None):
+    b = ph.Batch(ctx, cfg, offs)
+    dk = hip_rt.DeviceBuffer.from_numpy(keys)
+    dp = hip_rt.DeviceBuffer(max(b.total_pilots, 1) * 4)
+    do = hip_rt.DeviceBuffer(b.total_offsets * 4)
+    ds = hip_rt.DeviceBuffer(4)
+    outs = []
+    for r in range(reps):
+        dp.zero()
+        do.zero()
+        b.launch(dk, dp, do, ds, graph=graph)
+        hip_rt.synchronize()
+        assert int(ds.to_numpy(np.uint32)[0]) == 0
+        outs.append((dp.to_numpy(np.uint32, b.total_pilots), do.to_numpy(np.uint32, b.total_offsets)))
+        if between is not None:
+            between()
+    return b, outs
+
+
+def test_batch_sets_equal_individual_builds(ctx):
+    cfg = ph.make_config(7.0, 2048)
+    sizes, offs, keys = make_sets(120, 1e3, 2e5, 1, extra=(0, 1, 2048, 2049))
+    b, [(pil, off)] = run_batch(ctx, cfg, offs, keys)
+    for s in range(sizes.size):
+        sk = keys[int(offs[s]):int(offs[s + 1])]
+        r = ctx.build_u64(sk, cfg)
+        bp, bo = b.split(pil, off, s)
+        assert np.array_equal(bo, r.offsets), s
+        assert np.array_equal(bp, r.pilots), s
+    for s in (0, 7, sizes.size - 1):  # and the oracle for a few sets
+        sk = keys[int(offs[s]):int(offs[s + 1])]
+        st, opil, ooff = po.build_keys(po.hash_u64(sk, 0), 7.0, 2048)
+        bp, bo = b.split(pil, off, s)
+        assert np.array_equal(bp, opil) and np.array_equal(bo, ooff)
+    b.close()
+
+
+def test_batch_graph_replays_match_eager(ctx):
+    cfg = ph.make_config(8.0, 2500)
+    sizes, offs, keys = make_sets(60, 1e3, 1e5, 2)
+    _, [eager] = run_batch(ctx, cfg, offs, keys)
+    other = po.splitmix64(5, 300_000)
+
+    def eager_work():  # unrelated build on the same context between replays
+        ctx.build_u64(other, ph.make_config(7.0, 2048))
+
+    b, outs = run_batch(ctx, cfg, offs, keys, graph=True, reps=4, between=eager_work)
+    for pil, off in outs:
+        assert np.array_equal(pil, eager[0]) and np.array_equal(off, eager[1])
+    b.close()
+
+
+def test_batch_single_set_equals_plain(ctx):
+    cfg = ph.make_config(7.5, 2048)
+    keys = po.splitmix64(3, 1_500_000)
+    offs = np.array([0, keys.size], np.uint64)
+    b, [(pil, off)] = run_batch(ctx, cfg, offs, keys)
+    r = ctx.build_u64(keys, cfg)
+    assert np.array_equal(pil, r.pilots) and np.array_equal(off, r.offsets)
+    b.close()

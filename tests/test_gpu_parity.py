@@ -178,7 +178,7 @@ def test_device_build_matches_host_build(ctx):
     dk = hip_rt.DeviceBuffer(n * 8)
     ph.gen_splitmix64_device(2, n, dk)
     dp, do, ds = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
-    ctx.build_u64_device(dk, n, cfg, dp, do, ds)
+    ctx.build_u64_device(dk, n, cfg, ph.device_outputs(dp, do), ds)
     hip_rt.synchronize()
     assert int(ds.to_numpy(np.uint32)[0]) == 0
     r = ctx.build_u64(po.splitmix64(2, n), cfg)
@@ -195,14 +195,53 @@ def test_device_bijection_100m(ctx):
     dk = hip_rt.DeviceBuffer(n * 8)
     ph.gen_splitmix64_device(2, n, dk)
     dp, do, ds = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
-    ctx.build_u64_device(dk, n, cfg, dp, do, ds)
+    mphf = ph.device_outputs(dp, do)
+    ctx.build_u64_device(dk, n, cfg, mphf, ds)
     hip_rt.synchronize()
     assert int(ds.to_numpy(np.uint32)[0]) == 0
     off = do.to_numpy(np.uint32)
     assert int(off[-1]) == n and np.all(np.diff(off.astype(np.int64)) <= P + P // 2)
     out = hip_rt.DeviceBuffer(n * 4)
-    ctx.eval_u64_device(dk, n, cfg, np_, dp, do, out)
+    ctx.eval_u64_device(dk, n, cfg, np_, mphf, out)
     bm, bad = hip_rt.DeviceBuffer((n + 31) // 32 * 4), hip_rt.DeviceBuffer(8)
     ctx.check_bijection_device(out, n, n, bm, bad)
     hip_rt.synchronize()
+    assert int(bad.to_numpy(np.uint64)[0]) == 0
+
+
+@pytest.mark.parametrize("n,lam,P,alpha,kseed", [
+    (1_000_000, 7.5, 2048, 0.99, 3),   # the 1B-key configuration's lambda / alpha
+    (400_000, 6.0, 2048, 0.99, 4),
+    (300_000, 8.0, 2500, 0.95, 5),
+    (100_000, 7.0, 1024, 0.5, 6),
+])
+def test_alpha_remap_bit_exact(ctx, n, lam, P, alpha, kseed):
+    keys = po.splitmix64(kseed, n)
+    r = ctx.build_u64(keys, ph.make_config(lam, P, 0, alpha))
+    hk = po.hash_u64(keys, 0)
+    st, pil, off, rm, roff = po.build_keys_alpha(hk, lam, P, alpha)
+    assert st == 0
+    assert np.array_equal(r.offsets, off) and np.array_equal(r.pilots, pil)
+    assert np.array_equal(r.remap_offsets, roff) and np.array_equal(r.remap, rm)
+    q = po.query_keys_alpha(hk, r.pilots, r.offsets, r.remap, r.remap_offsets, lam, P, alpha)
+    assert int(q.max()) < n and np.unique(q).size == n
+
+
+def test_alpha_device_eval_bijection(ctx):
+    n, lam, P, alpha = 5_000_000, 7.5, 2048, 0.99
+    cfg = ph.make_config(lam, P, 0, alpha)
+    np_, B = ph.sizes(cfg, n)
+    rcap = ph.remap_capacity(cfg, n)
+    dk = hip_rt.DeviceBuffer(n * 8)
+    ph.gen_splitmix64_device(31, n, dk)
+    dp, do = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4)
+    drm, dro, ds = hip_rt.DeviceBuffer(rcap * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
+    mphf = ph.device_outputs(dp, do, drm, dro)
+    ctx.build_u64_device(dk, n, cfg, mphf, ds)
+    out = hip_rt.DeviceBuffer(n * 4)
+    ctx.eval_u64_device(dk, n, cfg, np_, mphf, out)
+    bm, bad = hip_rt.DeviceBuffer((n + 31) // 32 * 4), hip_rt.DeviceBuffer(8)
+    ctx.check_bijection_device(out, n, n, bm, bad)
+    hip_rt.synchronize()
+    assert int(ds.to_numpy(np.uint32)[0]) == 0
     assert int(bad.to_numpy(np.uint64)[0]) == 0
