@@ -68,6 +68,177 @@ def cpu_baseline(n_sample: int, lam: float, P: int, kseed: int) -> dict:
     }
 
 
+def bench_batch(args) -> int:
+    """Batched builds of `sets` independent key sets (sizes log-uniform in
+    [1e3, 1e6], seeded; keys of set s: splitmix64 seed 1000 + s), lambda 7,
+    P 2048, against building the same sets one at a time."""
+    rng = np.random.default_rng(4)
+    sizes = np.floor(np.exp(rng.uniform(np.log(1e3), np.log(1e6), args.sets))).astype(np.int64)
+    offs = np.zeros(args.sets + 1, np.uint64)
+    offs[1:] = np.cumsum(sizes)
+    n = int(offs[-1])
+    cfg = ph.make_config(7.0, 2048, 0)
+    ctx = ph.Context(0)
+    stream = hip_rt.Stream()
+    dk = hip_rt.DeviceBuffer(n * 8)
+    for s in range(args.sets):
+        ph.gen_splitmix64_device(1000 + s, int(sizes[s]), dk.ptr + 8 * int(offs[s]), stream)
+    batch = ph.Batch(ctx, cfg, offs)
+    dp_b = hip_rt.DeviceBuffer(batch.total_pilots * 4)
+    do_b = hip_rt.DeviceBuffer(batch.total_offsets * 4)
+    dp_p = hip_rt.DeviceBuffer(batch.total_pilots * 4)
+    do_p = hip_rt.DeviceBuffer(batch.total_offsets * 4)
+    ds = hip_rt.DeviceBuffer(4)
+    stream.synchronize()
+
+    def plain_once():
+        for s in range(args.sets):
+            pb = int(batch.part_base[s])
+            out = ph.device_outputs(dp_p.ptr + 4 * batch.B * pb, do_p.ptr + 4 * (pb + s))
+            ctx.build_u64_device(dk.ptr + 8 * int(offs[s]), int(sizes[s]), cfg, out, ds, stream)
+            stream.synchronize()  # one synchronous build per set, as MPHFbuilder::build
+
+    def timed(fn, reps):
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        stream.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    plain_once()  # warm-up (allocations)
+    t_plain = timed(plain_once, 1)
+    batch.launch(dk, dp_b, do_b, ds, stream)  # warm-up
+    t_eager = timed(lambda: batch.launch(dk, dp_b, do_b, ds, stream), max(1, args.steps))
+    batch.launch(dk, dp_b, do_b, ds, stream, graph=True)  # capture
+    t_graph = timed(lambda: batch.launch(dk, dp_b, do_b, ds, stream, graph=True), max(1, args.steps))
+    status = int(ds.to_numpy(np.uint32)[0])
+    same = bool(np.array_equal(dp_b.to_numpy(np.uint32), dp_p.to_numpy(np.uint32)) and
+                np.array_equal(do_b.to_numpy(np.uint32), do_p.to_numpy(np.uint32)))
+    best = min(t_eager, t_graph)
+    out = {
+        "metric": "batched-vs-plain keys/s",
+        "value": round(n / best / 1e6, 2),
+        "unit": "Mkeys/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(best * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": f"synthetic: {args.sets} sets, sizes floor(exp(U[ln 1e3, ln 1e6])) (numpy seed 4), "
+                "keys of set s = splitmix64 seed 1000+s, generated in HBM",
+        "config": {"workload": f"batched build of {args.sets} independent u64 key sets, lambda 7.0, P 2048, "
+                               "alpha 1.0", "keys": n, "sets": args.sets},
+        "plain_one_at_a_time_mkeys_s": round(n / t_plain / 1e6, 2),
+        "batched_eager_mkeys_s": round(n / t_eager / 1e6, 2),
+        "batched_graph_mkeys_s": round(n / t_graph / 1e6, 2),
+        "speedup_vs_plain": round(t_plain / best, 2),
+        "identical_outputs": same,
+        "status": status,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    ctx.close()
+    return 0 if same and status == 0 else 1
+
+
+def bench_serve(args) -> int:
+    """A finite Poisson trace (mean 2000 requests/s) of build requests:
+    sizes log-uniform in [1e4, 1e8] (u64) / [1e4, 2e6] (strings of length
+    8..64), 70 % u64 / 30 % strings, lambda in {6,7,8}, alpha in {0.99, 1},
+    served under a 200 GB HBM budget, against the plain sequential path."""
+    rng = np.random.default_rng(5)
+    R = args.requests
+    arrivals = np.cumsum(rng.exponential(1 / 2000.0, R))
+    reqs = []
+    for i in range(R):
+        lam = float(rng.choice([6.0, 7.0, 8.0]))
+        alpha = float(rng.choice([0.99, 1.0]))
+        cfg = ph.make_config(lam, 2048, 0, alpha)
+        if rng.random() < 0.7:
+            n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e8))))
+            d = hip_rt.DeviceBuffer(n * 8)
+            ph.gen_splitmix64_device(20000 + i, n, d)
+            reqs.append(("u64", n, d, cfg))
+        else:
+            n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+            lens = rng.integers(8, 65, n)
+            offs = np.zeros(n + 1, np.uint64)
+            offs[1:] = np.cumsum(lens)
+            data = rng.integers(0x20, 0x7F, int(offs[-1]), dtype=np.uint8)
+            idx = np.arange(n, dtype=np.uint64)  # distinct: the first 8 bytes carry the index
+            starts = offs[:-1].astype(np.int64)
+            for b in range(8):
+                data[starts + b] = (0x30 + ((idx >> np.uint64(4 * b)) & np.uint64(0xF))).astype(np.uint8)
+            reqs.append(("bytes", n, (data, offs), cfg))
+    hip_rt.synchronize()
+    total = sum(r[1] for r in reqs)
+
+    # plain sequential path: one synchronous build at a time
+    ctx = ph.Context(0)
+    stream = hip_rt.Stream()
+    build_s = []
+    for kind, n, payload, cfg in reqs:
+        np_, B = ph.sizes(cfg, n)
+        rc = ph.remap_capacity(cfg, n)
+        dp, do, ds = hip_rt.DeviceBuffer(max(np_ * B, 1) * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
+        drm = hip_rt.DeviceBuffer(max(rc, 1) * 4) if rc else None
+        dro = hip_rt.DeviceBuffer((np_ + 1) * 4) if rc else None
+        outs = ph.device_outputs(dp, do, drm, dro)
+        if kind == "bytes":
+            dbytes = hip_rt.DeviceBuffer.from_numpy(payload[0])
+            doffs = hip_rt.DeviceBuffer.from_numpy(payload[1])
+        stream.synchronize()
+        t0 = time.perf_counter()
+        if kind == "u64":
+            ctx.build_u64_device(payload, n, cfg, outs, ds, stream)
+        else:
+            ctx.build_bytes_device(dbytes, doffs, n, cfg, outs, ds, stream)
+        stream.synchronize()
+        build_s.append(time.perf_counter() - t0)
+    # sequential FIFO queue over the same arrivals
+    done, lat = 0.0, []
+    for a, b in zip(arrivals, build_s):
+        done = max(done, a) + b
+        lat.append(done - a)
+    seq_span = done - arrivals[0]
+    ctx.close()
+
+    srv = ph.Server(slots=8, hbm_budget=200 << 30)
+    t0 = time.perf_counter()
+    ids = []
+    for (kind, n, payload, cfg), a in zip(reqs, arrivals):
+        now = time.perf_counter() - t0
+        if a > now:
+            time.sleep(a - now)
+        ids.append(srv.submit_u64(payload, cfg, n) if kind == "u64" else srv.submit_bytes(payload[0], payload[1], cfg))
+    statuses = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    for i in ids:
+        srv.release(i)
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "p99 latency",
+        "value": round(st["p99_ms"], 2),
+        "unit": "ms",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(st["span_s"] * 1e3, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": f"synthetic trace of {R} requests (numpy seed 5), u64 keys generated in HBM, strings on the host",
+        "config": {"workload": "build server: Poisson arrivals, mean 2000 req/s, 70% u64 log-U[1e4,1e8] / "
+                               "30% strings log-U[1e4,2e6] of length 8..64, lambda {6,7,8}, alpha {0.99,1}, "
+                               "200 GB HBM budget, 8 slots", "requests": R, "keys": total},
+        "p50_ms": round(st["p50_ms"], 2),
+        "p99_ms": round(st["p99_ms"], 2),
+        "served_mkeys_s": round(total / st["span_s"] / 1e6, 2),
+        "sequential_mkeys_s": round(total / seq_span / 1e6, 2),
+        "sequential_p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
+        "sequential_p99_ms": round(float(np.percentile(lat_ms, 99)), 2),
+        "failed": int(sum(1 for s in statuses if s)),
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if out["failed"] == 0 else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,12 +252,19 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--mode", choices=["plain", "batch", "serve"], default="plain")
+    ap.add_argument("--sets", type=int, default=4096)
+    ap.add_argument("--requests", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build path runs on a single device; --gpus {args.gpus} is not "
               "supported (sharding by partition range is described in DESIGN.md)", file=sys.stderr)
         return 2
 
+    if args.mode == "batch":
+        return bench_batch(args)
+    if args.mode == "serve":
+        return bench_serve(args)
     n, lam, P = args.n, args.lam, args.P
     alpha = args.alpha
     cfg = ph.make_config(lam, P, 0, alpha)
