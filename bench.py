@@ -79,6 +79,7 @@ def bench_batch(args) -> int:
     n = int(offs[-1])
     cfg = ph.make_config(7.0, 2048, 0)
     ctx = ph.Context(0)
+    tuning, _ = ctx.autotune(cfg, args.autotune or 8_000_000)  # launch shapes for this configuration
     stream = hip_rt.Stream()
     dk = hip_rt.DeviceBuffer(n * 8)
     for s in range(args.sets):
@@ -131,6 +132,7 @@ def bench_batch(args) -> int:
         "batched_eager_mkeys_s": round(n / t_eager / 1e6, 2),
         "batched_graph_mkeys_s": round(n / t_graph / 1e6, 2),
         "speedup_vs_plain": round(t_plain / best, 2),
+        "tuning": tuning,
         "identical_outputs": same,
         "status": status,
     }
@@ -255,6 +257,8 @@ def main() -> int:
     ap.add_argument("--mode", choices=["plain", "batch", "serve"], default="plain")
     ap.add_argument("--sets", type=int, default=4096)
     ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="sample keys for the launch-shape autotuner (0 = default shapes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build path runs on a single device; --gpus {args.gpus} is not "
@@ -271,6 +275,9 @@ def main() -> int:
     np_, B = ph.sizes(cfg, n)
     rcap = ph.remap_capacity(cfg, n)
     ctx = ph.Context(0)
+    tuning = None
+    if args.autotune:
+        tuning, _ = ctx.autotune(cfg, args.autotune)
     stream = hip_rt.Stream()
     dk = hip_rt.DeviceBuffer(n * 8)
     ph.gen_splitmix64_device(args.key_seed, n, dk, stream)
@@ -332,6 +339,8 @@ def main() -> int:
             "buckets_per_partition": B,
         },
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "tuning": ctx.tuning(),
+        "autotuned": bool(tuning),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
         "roofline": {
             "kernel": dom,

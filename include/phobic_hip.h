@@ -211,6 +211,25 @@ int phobic_server_release(phobic_server* server, uint64_t id);
 int phobic_server_stats_get(phobic_server* server, phobic_server_stats* stats);
 int phobic_server_destroy(phobic_server* server);
 
+/* ---- launch shapes and the autotuner ----
+ * search_waves: partitions (one wave64 each) per search workgroup (1,2,4,8);
+ * route_per: records per thread of a partitioning tile (8 or 16);
+ * max_fanout: bins per partitioning level (16..128).  Defaults {4, 16, 128}.
+ * phobic_autotune times every combination of {1,2,4,8} x {8,16} x
+ * {32,64,128} on `sample_keys` device-generated keys built with `cfg`,
+ * keeps the fastest on the context and returns it.  Results do not depend
+ * on the tuning. */
+typedef struct {
+  uint32_t search_waves;
+  uint32_t route_per;
+  uint32_t max_fanout;
+  uint32_t reserved;
+} phobic_tuning;
+int phobic_get_tuning(phobic_ctx* ctx, phobic_tuning* tuning);
+int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* tuning);
+int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_keys, phobic_tuning* best,
+                    float* best_ms);
+
 /* Stage timings of the last build on this context (device events). */
 int phobic_last_stage_times(phobic_ctx* ctx, phobic_stage_times* times);
 

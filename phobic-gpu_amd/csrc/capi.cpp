@@ -7,6 +7,7 @@
 // host-buffer entry points) copy inputs in and pilots / offsets out.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -64,6 +65,7 @@ struct phobic_ctx {
   DevBuf in_a, in_b, out_pilots, out_offsets, out_remap, out_roff;
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
+  phobic_tuning tuning{4, 16, kMaxFanout, 0};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
@@ -94,16 +96,17 @@ struct Plan {
   uint64_t G[kMaxLevels + 1];
 };
 
-// Fan-out f <= kMaxFanout and the fewest levels L with f^L >= np.
-void plan_levels(Plan* pl) {
+// Fan-out f <= max_fanout and the fewest levels L with f^L >= np; false
+// (plan untouched) when that needs more than kMaxLevels levels or a level-1
+// bin of more than 65535 partitions.
+bool plan_levels(Plan* pl, uint32_t max_fanout = kMaxFanout) {
   const uint32_t np = pl->np;
-  if (np <= kMaxFanout) {
-    pl->L = 1;
-    pl->fanout = np ? np : 1;
-  } else {
-    uint32_t L = 2;
-    while (std::pow((double)kMaxFanout, (double)L) < (double)np) L++;
-    uint32_t f = (uint32_t)std::ceil(std::pow((double)np, 1.0 / L));
+  uint32_t L = 1, f = np ? np : 1;
+  if (np > max_fanout) {
+    L = 2;
+    while (std::pow((double)max_fanout, (double)L) < (double)np) L++;
+    if (L > kMaxLevels) return false;
+    f = (uint32_t)std::ceil(std::pow((double)np, 1.0 / L));
     auto reach = [&](uint32_t ff) {
       uint64_t r = 1;
       for (uint32_t i = 0; i < L; i++) r *= ff;
@@ -111,14 +114,18 @@ void plan_levels(Plan* pl) {
     };
     while (f > 2 && reach(f - 1) >= np) f--;
     while (reach(f) < np) f++;
-    pl->L = L;
-    pl->fanout = f;
   }
-  for (uint32_t l = 1; l <= pl->L; l++) {
+  uint64_t G[kMaxLevels + 1] = {};
+  for (uint32_t l = 1; l <= L; l++) {
     uint64_t g = 1;
-    for (uint32_t i = l; i < pl->L; i++) g *= pl->fanout;
-    pl->G[l] = g;
+    for (uint32_t i = l; i < L; i++) g *= f;
+    G[l] = g;
   }
+  if (G[1] > 65535) return false;
+  pl->L = L;
+  pl->fanout = f;
+  for (uint32_t l = 0; l <= kMaxLevels; l++) pl->G[l] = G[l];
+  return true;
 }
 
 int validate(const phobic_config* cfg, uint64_t n, Plan* pl) {
@@ -136,10 +143,9 @@ int validate(const phobic_config* cfg, uint64_t n, Plan* pl) {
   if (!(alpha >= 0.5 && alpha <= 1.0)) return PHOBIC_E_INVALID_CONFIG;
   pl->a_ppm = (uint32_t)std::llround(alpha * 1e6);
   pl->tcap = table_size(pl->cap, pl->a_ppm);
-  if (search_lds_bytes(pl->tcap, pl->B) > 64 * 1024) return PHOBIC_E_INVALID_CONFIG;
+  if (search_lds_bytes(pl->tcap, pl->B, 1) > 64 * 1024) return PHOBIC_E_INVALID_CONFIG;
   if (pl->np > (1u << 21)) return PHOBIC_E_INVALID_CONFIG;  // partition index inside a bin must fit 16 bits
-  plan_levels(pl);
-  if (pl->L > kMaxLevels || pl->G[1] > 65535) return PHOBIC_E_INVALID_CONFIG;
+  if (!plan_levels(pl)) return PHOBIC_E_INVALID_CONFIG;
   return PHOBIC_OK;
 }
 
@@ -212,9 +218,24 @@ Layout layout_of(const Plan& pl) {
   return ly;
 }
 
+// The plan with the context's tuning applied (levels for its fan-out).
+Plan tuned(const phobic_ctx* ctx, Plan pl) {
+  if (!plan_levels(&pl, ctx->tuning.max_fanout)) plan_levels(&pl, kMaxFanout);  // validated plan kept
+  return pl;
+}
+
+// Partitions per search workgroup: the tuned value, reduced until the
+// workgroup's LDS fits 64 KiB.
+uint32_t search_waves(const phobic_ctx* ctx, const Plan& pl) {
+  uint32_t w = ctx->tuning.search_waves;
+  while (w > 1 && search_lds_bytes(pl.tcap, pl.B, w) > 64 * 1024) w /= 2;
+  return w;
+}
+
 // Allocations and the fulcrum upload: everything a build needs before its
 // kernels are enqueued (not capturable into a graph).
-int prepare_build(phobic_ctx* ctx, const phobic_config* cfg, const Plan& pl, hipStream_t s) {
+int prepare_build(phobic_ctx* ctx, const phobic_config* cfg, const Plan& pl0, hipStream_t s) {
+  const Plan pl = tuned(ctx, pl0);
   TRY(upload_fulcrums(ctx, cfg, s));
   const Layout ly = layout_of(pl);
   TRY_HIP(ctx->pcount.ensure(((uint64_t)pl.np + pl.fanout) * 4));
@@ -228,8 +249,10 @@ int prepare_build(phobic_ctx* ctx, const phobic_config* cfg, const Plan& pl, hip
 
 // Kernel launches of one build (fills included, no allocation, no host
 // copies): capturable into a hipGraph when `timed` is false.
-int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_config* cfg, const Plan& pl,
+int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_config* cfg, const Plan& pl0,
                  const phobic_outputs& d_out, uint32_t* d_status, hipStream_t s, const BatchInfo& batch, bool timed) {
+  const Plan pl = tuned(ctx, pl0);
+  const uint32_t per = ctx->tuning.route_per;
   uint32_t* d_pilots = d_out.pilots;
   const bool remap = pl.a_ppm < 1000000u;
   const uint32_t np = pl.np, L = pl.L, f = pl.fanout;
@@ -261,7 +284,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
   ha.sets = batch.sets;
   ha.set_key_off = batch.set_key_off;
   ha.set_part_base = batch.set_part_base;
-  TRY_HIP(launch_hash_route(kind, ha, s));
+  TRY_HIP(launch_hash_route(kind, ha, per, s));
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[1], s));
 
   for (uint32_t l = 2; l <= L; l++) {
@@ -271,7 +294,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
     sp.in_meta = ctx->rec_meta[out_buf(l - 1)].as<uint32_t>();
     sp.parent_count = counts(l - 1);
     sp.parent_cap = pl.G[l - 1] * pl.cap;
-    sp.slices = (uint32_t)((sp.parent_cap + route_tile_records() - 1) / route_tile_records());
+    sp.slices = (uint32_t)((sp.parent_cap + route_tile_records(per) - 1) / route_tile_records(per));
     sp.fanout = f;
     sp.G_child = (uint32_t)pl.G[l];
     sp.cap = pl.cap;
@@ -279,7 +302,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
     sp.out_lo = ctx->rec_lo[out_buf(l)].as<uint64_t>();
     sp.out_meta = ctx->rec_meta[out_buf(l)].as<uint32_t>();
     sp.status = d_status;
-    TRY_HIP(launch_split_level(sp, parents, s));
+    TRY_HIP(launch_split_level(sp, parents, per, s));
   }
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[2], s));
 
@@ -309,7 +332,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
   ra.status = d_status;
   ra.sets = batch.sets;
   ra.set_part_base = batch.set_part_base;
-  TRY_HIP(launch_partition_search(ra, s));
+  TRY_HIP(launch_partition_search(ra, search_waves(ctx, pl), s));
   if (timed) {
     TRY_HIP(hipEventRecord(ctx->ev[4], s));
     ctx->timed = true;
@@ -499,8 +522,7 @@ int phobic_batch_create(phobic_ctx* ctx, const phobic_config* cfg, const uint64_
     pl.n = n;
     pl.np = (uint32_t)total_parts;
     if (pl.np > (1u << 21)) return PHOBIC_E_INVALID_CONFIG;
-    plan_levels(&pl);
-    if (pl.L > kMaxLevels || pl.G[1] > 65535) return PHOBIC_E_INVALID_CONFIG;
+    if (!plan_levels(&pl)) return PHOBIC_E_INVALID_CONFIG;
   }
   phobic_batch* b = new (std::nothrow) phobic_batch();
   if (!b) return PHOBIC_E_OOM;
@@ -723,6 +745,82 @@ int phobic_build_keys_device(phobic_ctx* ctx, const phobic_key* d_keys, uint64_t
   src.keys = reinterpret_cast<const uint4*>(d_keys);
   if (n && !d_keys) return PHOBIC_E_INVALID_CONFIG;
   return device_build(ctx, kKindKey, src, n, cfg, d_out, d_status, stream);
+}
+
+int phobic_get_tuning(phobic_ctx* ctx, phobic_tuning* t) {
+  if (!ctx || !t) return PHOBIC_E_INVALID_CONFIG;
+  *t = ctx->tuning;
+  return PHOBIC_OK;
+}
+
+int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* t) {
+  if (!ctx || !t) return PHOBIC_E_INVALID_CONFIG;
+  const uint32_t w = t->search_waves, per = t->route_per, f = t->max_fanout;
+  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 16) || f < 16 || f > kMaxFanout)
+    return PHOBIC_E_INVALID_CONFIG;
+  ctx->tuning = *t;
+  ctx->tuning.reserved = 0;
+  return PHOBIC_OK;
+}
+
+int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_keys, phobic_tuning* best,
+                    float* best_ms) {
+  if (!ctx || sample_keys == 0) return PHOBIC_E_INVALID_CONFIG;
+  Plan pl;
+  TRY(validate(cfg, sample_keys, &pl));
+  TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  uint64_t rcap = 0;
+  TRY(phobic_remap_capacity(cfg, sample_keys, &rcap));
+  // sample keys and outputs, freed on return
+  DevBuf keys, pil, off, rm, ro, st;
+  auto cleanup = [&]() {
+    for (DevBuf* b : {&keys, &pil, &off, &rm, &ro, &st}) b->release();
+  };
+  hipError_t e = keys.ensure(sample_keys * 8);
+  if (e == hipSuccess) e = pil.ensure((uint64_t)pl.np * pl.B * 4);
+  if (e == hipSuccess) e = off.ensure(((uint64_t)pl.np + 1) * 4);
+  if (e == hipSuccess) e = rm.ensure(rcap * 4 + 4);
+  if (e == hipSuccess) e = ro.ensure(((uint64_t)pl.np + 1) * 4);
+  if (e == hipSuccess) e = st.ensure(4);
+  if (e == hipSuccess) e = launch_splitmix64(0x5eed, sample_keys, keys.as<uint64_t>(), s);
+  if (e != hipSuccess) {
+    cleanup();
+    return hip_status(e);
+  }
+  KeySource src{};
+  src.u64 = keys.as<uint64_t>();
+  phobic_outputs outs{pil.as<uint32_t>(), off.as<uint32_t>(), rcap ? rm.as<uint32_t>() : nullptr,
+                      rcap ? ro.as<uint32_t>() : nullptr};
+  const phobic_tuning saved = ctx->tuning;
+  phobic_tuning win = saved;
+  float win_ms = 1e30f;
+  for (uint32_t w : {1u, 2u, 4u, 8u})
+    for (uint32_t per : {8u, 16u})
+      for (uint32_t f : {32u, 64u, 128u}) {
+        ctx->tuning = phobic_tuning{w, per, f, 0};
+        float ms = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {  // first run warms up
+          int r = enqueue_build(ctx, kKindU64, src, cfg, pl, outs, st.as<uint32_t>(), s);
+          if (r == PHOBIC_OK && hipStreamSynchronize(s) != hipSuccess) r = PHOBIC_E_HIP;
+          if (r != PHOBIC_OK) {
+            ctx->tuning = saved;
+            cleanup();
+            return r;
+          }
+          float t = 0;
+          if (rep && hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[kStages]) == hipSuccess) ms = std::min(ms, t);
+        }
+        if (ms < win_ms) {
+          win_ms = ms;
+          win = ctx->tuning;
+        }
+      }
+  ctx->tuning = win;
+  cleanup();
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return PHOBIC_OK;
 }
 
 int phobic_last_stage_times(phobic_ctx* ctx, phobic_stage_times* times) {

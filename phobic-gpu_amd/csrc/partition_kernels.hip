@@ -26,10 +26,11 @@ namespace phobic {
 namespace {
 
 constexpr uint32_t kThreads = 256;
-constexpr uint32_t kPer = 16;  // records per thread per tile
-constexpr uint32_t kTile = kThreads * kPer;
 
+// kPer records per thread, kPer * 256 per tile
+template <uint32_t kPer>
 struct TileLds {
+  static constexpr uint32_t kTile = kThreads * kPer;
   uint64_t lo[kTile];
   uint32_t meta[kTile];
   uint8_t child[kTile];
@@ -51,7 +52,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 
 // Routes the block's tile (record k of thread t valid when valid[k]) to
 // child bins child_first + child[k] of capacity child_cap.
-__device__ __forceinline__ void route_tile(TileLds& s, const uint64_t (&lo)[kPer], const uint32_t (&meta)[kPer],
+template <uint32_t kPer>
+__device__ __forceinline__ void route_tile(TileLds<kPer>& s, const uint64_t (&lo)[kPer], const uint32_t (&meta)[kPer],
                                            const uint32_t (&child)[kPer], const bool (&valid)[kPer],
                                            uint32_t fanout, uint64_t child_first, uint64_t child_cap,
                                            uint32_t* __restrict__ child_count, uint64_t* __restrict__ out_lo,
@@ -124,9 +126,10 @@ __device__ __forceinline__ U128 hash_key(const KeySource& src, uint64_t i, uint6
 
 // Level 1: hash raw keys, map to (partition, bucket) and route to the
 // level-1 bins (partitions [c*G, (c+1)*G)).
-template <int KIND>
+template <int KIND, uint32_t kPer>
 __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
-  __shared__ TileLds s;
+  constexpr uint32_t kTile = kThreads * kPer;
+  __shared__ TileLds<kPer> s;
   __shared__ uint32_t fl[kFulcs];
   for (uint32_t i = threadIdx.x; i < kFulcs; i += kThreads) fl[i] = a.fulcs[i];
   __syncthreads();
@@ -177,8 +180,10 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
 
 // Level l > 1: split every parent bin (G_parent partitions) into `fanout`
 // child bins of G_child partitions.  Block (b, s) handles tile s of parent b.
+template <uint32_t kPer>
 __global__ __launch_bounds__(kThreads) void k_split_level(SplitArgs a) {
-  __shared__ TileLds s;
+  constexpr uint32_t kTile = kThreads * kPer;
+  __shared__ TileLds<kPer> s;
   const uint32_t parent = blockIdx.x / a.slices;
   const uint64_t tile0 = (uint64_t)(blockIdx.x % a.slices) * kTile;
   const uint32_t cnt = min(a.parent_count[parent], (uint32_t)min((uint64_t)0xFFFFFFFFu, a.parent_cap));
@@ -207,23 +212,32 @@ __global__ __launch_bounds__(kThreads) void k_split_level(SplitArgs a) {
              a.child_count, a.out_lo, a.out_meta, a.status);
 }
 
-uint32_t route_tile_records() { return kTile; }
+uint32_t route_tile_records(uint32_t per) { return kThreads * (per == 8 ? 8 : 16); }
 
-hipError_t launch_hash_route(int kind, const HashRouteArgs& a, hipStream_t s) {
-  if (a.n == 0) return hipSuccess;
+template <uint32_t kPer>
+static hipError_t hash_route_per(int kind, const HashRouteArgs& a, hipStream_t s) {
+  constexpr uint32_t kTile = kThreads * kPer;
   const uint64_t ntiles = (a.n + kTile - 1) / kTile;
-  const uint32_t grid = (uint32_t)(ntiles < 2048 ? ntiles : 2048);
+  const uint32_t grid = (uint32_t)(ntiles < 4096 ? ntiles : 4096);
   switch (kind) {
-    case kKindU64: hipLaunchKernelGGL(k_hash_route<kKindU64>, dim3(grid), dim3(kThreads), 0, s, a); break;
-    case kKindBytes: hipLaunchKernelGGL(k_hash_route<kKindBytes>, dim3(grid), dim3(kThreads), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_hash_route<kKindKey>, dim3(grid), dim3(kThreads), 0, s, a); break;
+    case kKindU64: hipLaunchKernelGGL((k_hash_route<kKindU64, kPer>), dim3(grid), dim3(kThreads), 0, s, a); break;
+    case kKindBytes: hipLaunchKernelGGL((k_hash_route<kKindBytes, kPer>), dim3(grid), dim3(kThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_hash_route<kKindKey, kPer>), dim3(grid), dim3(kThreads), 0, s, a); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_split_level(const SplitArgs& a, uint32_t parents, hipStream_t s) {
+hipError_t launch_hash_route(int kind, const HashRouteArgs& a, uint32_t per, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  return per == 8 ? hash_route_per<8>(kind, a, s) : hash_route_per<16>(kind, a, s);
+}
+
+hipError_t launch_split_level(const SplitArgs& a, uint32_t parents, uint32_t per, hipStream_t s) {
   if (parents == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_split_level, dim3(parents * a.slices), dim3(kThreads), 0, s, a);
+  if (per == 8)
+    hipLaunchKernelGGL(k_split_level<8>, dim3(parents * a.slices), dim3(kThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_split_level<16>, dim3(parents * a.slices), dim3(kThreads), 0, s, a);
   return hipGetLastError();
 }
 

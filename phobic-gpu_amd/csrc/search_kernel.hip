@@ -31,7 +31,6 @@
 namespace phobic {
 namespace {
 
-constexpr uint32_t kWavesPerGroup = 4;
 constexpr uint32_t kMaxBucket = 256;  // sort_sum.comp BINS, search.comp MAX_BUCKET_SIZE
 
 // LDS words of one partition for tables of at most tcap slots
@@ -51,6 +50,18 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
+
+// Orders this wave's LDS accesses (a wave's LDS operations execute in
+// order; this waits for them and stops the compiler from moving memory
+// operations across).  Pending global loads are not waited for.
+__device__ __forceinline__ void lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// Identical keys collide for every pilot; a bucket still colliding at this
+// pilot is checked for them once (the outcome equals checking at pilot 0).
+constexpr uint32_t kDupCheckPilot = 1024;
+
+// records loaded per lane per step of the partition prologue
+constexpr uint32_t kLoadBatch = 8;
 
 template <int CTRL, int ROWS, bool BOUND>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
@@ -103,6 +114,7 @@ __device__ __forceinline__ uint64_t low_mask(uint32_t bits) {  // bits >= 64 -> 
 
 }  // namespace
 
+template <uint32_t kWavesPerGroup>
 __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(SearchArgs a) {
   extern __shared__ __align__(16) uint32_t smem[];
   const uint32_t lane = __lane_id();
@@ -151,11 +163,21 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
     pil[i] = 0;
   }
   for (uint32_t i = lane; i < 256; i += 64) hist[i] = 0;
-  wave_sync();
+  lds_sync();
 
-  // 1. bucket sizes
-  for (uint32_t i = lane; i < ps; i += 64) atomicAdd(&bcnt[a.rec_meta[base + i] & 0xFFFFu], 1u);
-  wave_sync();
+  // 1. bucket sizes (kLoadBatch loads in flight per lane)
+  for (uint32_t i0 = 0; i0 < ps; i0 += 64 * kLoadBatch) {
+    uint32_t m[kLoadBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kLoadBatch; j++) {
+      const uint32_t i = i0 + lane + 64 * j;
+      m[j] = i < ps ? a.rec_meta[base + i] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kLoadBatch; j++)
+      if (m[j] != 0xFFFFFFFFu) atomicAdd(&bcnt[m[j] & 0xFFFFu], 1u);
+  }
+  lds_sync();
   bool over = false;
   for (uint32_t b = lane; b < B; b += 64) {
     const uint32_t s = bcnt[b];
@@ -167,7 +189,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
     for (uint32_t b = lane; b < B; b += 64) out_pilots[out_p + (uint64_t)b * out_np] = 0;
     return;
   }
-  wave_sync();
+  lds_sync();
 
   // 2a. bin starts: exclusive scan over t = 256 - size (largest first)
   uint32_t m;
@@ -187,7 +209,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
       run += v[j];
     }
   }
-  wave_sync();
+  lds_sync();
 
   // 2b. ranks: within one size, larger bucket index first
   for (uint32_t c = 0; c < B; c += 64) {
@@ -206,13 +228,13 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
         order[bv + __popcll(same & lanemask_lt())] = (uint32_t)idx;
         act = false;
       }
-      wave_sync();
+      lds_sync();
       if (lane == leader) hist[t] = bv + __popcll(same);
-      wave_sync();
+      lds_sync();
       am &= ~same;
     }
   }
-  wave_sync();
+  lds_sync();
 
   // 3. bucket start offsets
   {
@@ -226,16 +248,28 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
       carry += __builtin_amdgcn_readlane(incl, 63);
     }
   }
-  wave_sync();
+  lds_sync();
 
   // 4. keys in processing order into this partition's slice of `sorted`
   uint64_t* __restrict__ sorted = a.sorted + base;
-  for (uint32_t i = lane; i < ps; i += 64) {
-    const uint32_t b = a.rec_meta[base + i] & 0xFFFFu;
-    const uint32_t pos = bstart[b] + atomicAdd(&bcur[b], 1u);
-    sorted[pos] = a.rec_lower[base + i];
+  for (uint32_t i0 = 0; i0 < ps; i0 += 64 * kLoadBatch) {
+    uint32_t m[kLoadBatch];
+    uint64_t v[kLoadBatch];
+#pragma unroll
+    for (uint32_t j = 0; j < kLoadBatch; j++) {
+      const uint32_t i = i0 + lane + 64 * j;
+      m[j] = i < ps ? a.rec_meta[base + i] : 0xFFFFFFFFu;
+      v[j] = i < ps ? a.rec_lower[base + i] : 0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kLoadBatch; j++) {
+      if (m[j] != 0xFFFFFFFFu) {
+        const uint32_t b = m[j] & 0xFFFFu;
+        sorted[bstart[b] + atomicAdd(&bcur[b], 1u)] = v[j];
+      }
+    }
   }
-  wave_sync();
+  wave_sync();  // the sorted slice is read back by other lanes
 
   // 5. pilot search over a table of tm slots (tm = ps unless alpha < 1)
   const uint32_t tm = table_size(ps, a.a_ppm);
@@ -297,15 +331,15 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
             c |= (atomicOr(&coll[pos[j] >> 5], msk) & msk) != 0;
           }
         }
-        wave_sync();
+        lds_sync();
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           const uint32_t k = lane + 64 * j;
           if (k < s) coll[pos[j] >> 5] = 0;
         }
-        wave_sync();
+        lds_sync();
         if (__any(c)) {
-          if (pilot == 0) {  // identical keys collide for every pilot
+          if (pilot == kDupCheckPilot) {
             bool dup = false;
             for (uint32_t k2 = 0; k2 < s; k2++) {
               const uint64_t kv = sorted[cst + k2];
@@ -368,7 +402,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
       }
     }
     if (lane == 0) pil[cb] = dsel + tm * pilot;
-    wave_sync();
+    lds_sync();
 #pragma unroll
     for (int j = 0; j < 4; j++) cur[j] = nxt[j];
     cs = ns;
@@ -376,7 +410,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
     cb = nb;
   }
   if (err && lane == 0) atomicOr(a.status, 1u << err);
-  wave_sync();
+  lds_sync();
 
   // 6. alpha < 1: the k-th used slot of [ps, tm) stands for the k-th free
   // slot of [0, ps).  R[0..K) first receives the free slots in order, then
@@ -424,12 +458,18 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
   for (uint32_t b = lane; b < B; b += 64) out_pilots[out_p + (uint64_t)b * out_np] = pil[b];
 }
 
-size_t search_lds_bytes(uint32_t tcap, uint32_t B) { return (size_t)lds_words(tcap, B) * 4 * kWavesPerGroup; }
+size_t search_lds_bytes(uint32_t tcap, uint32_t B, uint32_t waves) { return (size_t)lds_words(tcap, B) * 4 * waves; }
 
-hipError_t launch_partition_search(const SearchArgs& a, hipStream_t s) {
+hipError_t launch_partition_search(const SearchArgs& a, uint32_t waves, hipStream_t s) {
   if (a.np == 0) return hipSuccess;
-  const uint32_t groups = (a.np + kWavesPerGroup - 1) / kWavesPerGroup;
-  hipLaunchKernelGGL(k_partition_search, dim3(groups), dim3(64 * kWavesPerGroup), search_lds_bytes(a.tcap, a.B), s, a);
+  const uint32_t groups = (a.np + waves - 1) / waves;
+  const size_t lds = search_lds_bytes(a.tcap, a.B, waves);
+  switch (waves) {
+    case 1: hipLaunchKernelGGL(k_partition_search<1>, dim3(groups), dim3(64), lds, s, a); break;
+    case 2: hipLaunchKernelGGL(k_partition_search<2>, dim3(groups), dim3(128), lds, s, a); break;
+    case 8: hipLaunchKernelGGL(k_partition_search<8>, dim3(groups), dim3(512), lds, s, a); break;
+    default: hipLaunchKernelGGL(k_partition_search<4>, dim3(groups), dim3(256), lds, s, a); break;
+  }
   return hipGetLastError();
 }
 

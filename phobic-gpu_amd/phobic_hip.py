@@ -60,6 +60,18 @@ class Outputs(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("search_waves", ctypes.c_uint32),
+        ("route_per", ctypes.c_uint32),
+        ("max_fanout", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"search_waves": self.search_waves, "route_per": self.route_per, "max_fanout": self.max_fanout}
+
+
 class StageTimes(ctypes.Structure):
     _fields_ = [
         ("count", ctypes.c_uint32),
@@ -103,6 +115,9 @@ EXPORTS = [
     "phobic_server_release",
     "phobic_server_stats_get",
     "phobic_server_destroy",
+    "phobic_get_tuning",
+    "phobic_set_tuning",
+    "phobic_autotune",
     "phobic_last_stage_times",
     "phobic_eval_u64_device",
     "phobic_check_bijection_device",
@@ -152,6 +167,9 @@ def load() -> ctypes.CDLL:
         "phobic_server_release": (I, [P, U64]),
         "phobic_server_stats_get": (I, [P, P]),
         "phobic_server_destroy": (I, [P]),
+        "phobic_get_tuning": (I, [P, ctypes.POINTER(Tuning)]),
+        "phobic_set_tuning": (I, [P, ctypes.POINTER(Tuning)]),
+        "phobic_autotune": (I, [P, cfgp, U64, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]),
         "phobic_last_stage_times": (I, [P, tp]),
         "phobic_eval_u64_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
         "phobic_check_bijection_device": (I, [P, P, U64, U64, P, P, P]),
@@ -308,6 +326,24 @@ class Context:
                                                _ptr(d_status), _stream(stream))
         if st:
             raise PhobicError(st, "build_keys_device")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self.lib.phobic_get_tuning(self.h, ctypes.byref(t))
+        return t.as_dict()
+
+    def set_tuning(self, search_waves: int, route_per: int, max_fanout: int) -> None:
+        t = Tuning(search_waves, route_per, max_fanout, 0)
+        st = self.lib.phobic_set_tuning(self.h, ctypes.byref(t))
+        if st:
+            raise PhobicError(st, "set_tuning")
+
+    def autotune(self, cfg: Config, sample_keys: int) -> tuple[dict, float]:
+        t, ms = Tuning(), ctypes.c_float()
+        st = self.lib.phobic_autotune(self.h, ctypes.byref(cfg), sample_keys, ctypes.byref(t), ctypes.byref(ms))
+        if st:
+            raise PhobicError(st, "autotune")
+        return t.as_dict(), ms.value
 
     def last_stage_times(self) -> dict:
         t = StageTimes()

@@ -80,3 +80,26 @@ def test_batch_single_set_equals_plain(ctx):
     r = ctx.build_u64(keys, cfg)
     assert np.array_equal(pil, r.pilots) and np.array_equal(off, r.offsets)
     b.close()
+
+
+def test_tuning_does_not_change_results(ctx):
+    keys = po.splitmix64(21, 600_000)
+    cfg = ph.make_config(7.5, 2048)
+    saved = ctx.tuning()
+    ref = ctx.build_u64(keys, cfg)
+    try:
+        for w, per, f in [(1, 8, 32), (2, 16, 64), (8, 8, 128), (4, 16, 16)]:
+            ctx.set_tuning(w, per, f)
+            r = ctx.build_u64(keys, cfg)
+            assert np.array_equal(r.pilots, ref.pilots) and np.array_equal(r.offsets, ref.offsets), (w, per, f)
+    finally:
+        ctx.set_tuning(saved["search_waves"], saved["route_per"], saved["max_fanout"])
+
+
+def test_autotune_returns_valid_shape(ctx):
+    saved = ctx.tuning()
+    best, ms = ctx.autotune(ph.make_config(7.0, 2048), 2_000_000)
+    assert best["search_waves"] in (1, 2, 4, 8) and best["route_per"] in (8, 16)
+    assert best["max_fanout"] in (32, 64, 128) and ms > 0
+    assert ctx.tuning() == best
+    ctx.set_tuning(saved["search_waves"], saved["route_per"], saved["max_fanout"])
