@@ -241,6 +241,28 @@ def bench_serve(args) -> int:
     return 0 if out["failed"] == 0 else 1
 
 
+def bench_copy(args) -> int:
+    """A device-to-device copy of 1 GiB: the known-size reference for scaling
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings to HBM bytes."""
+    nbytes = 1 << 30
+    src, dst = hip_rt.DeviceBuffer(nbytes), hip_rt.DeviceBuffer(nbytes)
+    src.zero()
+    hip_rt.synchronize()
+    rt = hip_rt.rt()
+    e0, e1 = hip_rt.Event(), hip_rt.Event()
+    ms = []
+    for _ in range(max(1, args.steps)):
+        e0.record()
+        hip_rt.check(rt.hipMemcpyAsync(dst.ptr, src.ptr, nbytes, hip_rt.D2D, None), "hipMemcpyAsync")
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_ms(e1))
+    best = min(ms)
+    print(json.dumps({"mode": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "best_ms": round(best, 4),
+                      "gbs_read_plus_write": round(2 * nbytes / best / 1e6, 1)}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,7 +276,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=16_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
-    ap.add_argument("--mode", choices=["plain", "batch", "serve"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "batch", "serve", "copy"], default="plain")
     ap.add_argument("--sets", type=int, default=4096)
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--autotune", type=int, default=0,
@@ -269,6 +291,8 @@ def main() -> int:
         return bench_batch(args)
     if args.mode == "serve":
         return bench_serve(args)
+    if args.mode == "copy":
+        return bench_copy(args)
     n, lam, P = args.n, args.lam, args.P
     alpha = args.alpha
     cfg = ph.make_config(lam, P, 0, alpha)
