@@ -281,6 +281,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--autotune", type=int, default=0,
                     help="sample keys for the launch-shape autotuner (0 = default shapes)")
+    ap.add_argument("--search-mode", type=int, default=None, help="pilot-search variant mask (0..3)")
+    ap.add_argument("--search-waves", type=int, default=None, help="partitions per search workgroup")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build path runs on a single device; --gpus {args.gpus} is not "
@@ -302,6 +304,13 @@ def main() -> int:
     tuning = None
     if args.autotune:
         tuning, _ = ctx.autotune(cfg, args.autotune)
+    if args.search_mode is not None or args.search_waves is not None:
+        t = ctx.tuning()
+        if args.search_mode is not None:
+            t["search_mode"] = args.search_mode
+        if args.search_waves is not None:
+            t["search_waves"] = args.search_waves
+        ctx.set_tuning(**t)
     stream = hip_rt.Stream()
     dk = hip_rt.DeviceBuffer(n * 8)
     ph.gen_splitmix64_device(args.key_seed, n, dk, stream)

@@ -65,7 +65,7 @@ struct phobic_ctx {
   DevBuf in_a, in_b, out_pilots, out_offsets, out_remap, out_roff;
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
-  phobic_tuning tuning{4, 16, kMaxFanout, 0};
+  phobic_tuning tuning{2, 16, kMaxFanout, 0};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
@@ -332,6 +332,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
   ra.status = d_status;
   ra.sets = batch.sets;
   ra.set_part_base = batch.set_part_base;
+  ra.mode = ctx->tuning.search_mode;
   TRY_HIP(launch_partition_search(ra, search_waves(ctx, pl), s));
   if (timed) {
     TRY_HIP(hipEventRecord(ctx->ev[4], s));
@@ -756,10 +757,10 @@ int phobic_get_tuning(phobic_ctx* ctx, phobic_tuning* t) {
 int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* t) {
   if (!ctx || !t) return PHOBIC_E_INVALID_CONFIG;
   const uint32_t w = t->search_waves, per = t->route_per, f = t->max_fanout;
-  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 16) || f < 16 || f > kMaxFanout)
+  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 16) || f < 16 || f > kMaxFanout ||
+      t->search_mode > 3)
     return PHOBIC_E_INVALID_CONFIG;
   ctx->tuning = *t;
-  ctx->tuning.reserved = 0;
   return PHOBIC_OK;
 }
 
@@ -793,29 +794,69 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   phobic_outputs outs{pil.as<uint32_t>(), off.as<uint32_t>(), rcap ? rm.as<uint32_t>() : nullptr,
                       rcap ? ro.as<uint32_t>() : nullptr};
   const phobic_tuning saved = ctx->tuning;
+  // The partitioning shape (route_per, max_fanout) only changes the stages
+  // before the offsets and the search shape (search_waves, search_mode) only
+  // the search, so each is tuned on its own stages' device time.
+  auto time_stages = [&](const phobic_tuning& t, int e0, int e1, float* ms) -> int {
+    ctx->tuning = t;
+    *ms = 1e30f;
+    for (int rep = 0; rep < 3; rep++) {  // first run warms up
+      int r = enqueue_build(ctx, kKindU64, src, cfg, pl, outs, st.as<uint32_t>(), s);
+      if (r == PHOBIC_OK && hipStreamSynchronize(s) != hipSuccess) r = PHOBIC_E_HIP;
+      if (r != PHOBIC_OK) return r;
+      float t_ms = 0;
+      if (rep && hipEventElapsedTime(&t_ms, ctx->ev[e0], ctx->ev[e1]) == hipSuccess) *ms = std::min(*ms, t_ms);
+    }
+    return PHOBIC_OK;
+  };
   phobic_tuning win = saved;
-  float win_ms = 1e30f;
-  for (uint32_t w : {1u, 2u, 4u, 8u})
-    for (uint32_t per : {8u, 16u})
-      for (uint32_t f : {32u, 64u, 128u}) {
-        ctx->tuning = phobic_tuning{w, per, f, 0};
-        float ms = 1e30f;
-        for (int rep = 0; rep < 3; rep++) {  // first run warms up
-          int r = enqueue_build(ctx, kKindU64, src, cfg, pl, outs, st.as<uint32_t>(), s);
-          if (r == PHOBIC_OK && hipStreamSynchronize(s) != hipSuccess) r = PHOBIC_E_HIP;
-          if (r != PHOBIC_OK) {
-            ctx->tuning = saved;
-            cleanup();
-            return r;
-          }
-          float t = 0;
-          if (rep && hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[kStages]) == hipSuccess) ms = std::min(ms, t);
-        }
-        if (ms < win_ms) {
-          win_ms = ms;
-          win = ctx->tuning;
-        }
+  float part_ms = 1e30f, search_ms = 1e30f;
+  for (uint32_t per : {8u, 16u})
+    for (uint32_t f : {32u, 64u, 128u}) {
+      phobic_tuning t = win;
+      t.route_per = per;
+      t.max_fanout = f;
+      float ms;
+      const int r = time_stages(t, 0, kStages - 2, &ms);  // hash_route + partition_split
+      if (r != PHOBIC_OK) {
+        ctx->tuning = saved;
+        cleanup();
+        return r;
       }
+      if (ms < part_ms) {
+        part_ms = ms;
+        win.route_per = per;
+        win.max_fanout = f;
+      }
+    }
+  for (uint32_t w : {1u, 2u, 4u, 8u})
+    for (uint32_t mode : {0u, 1u, 2u, 3u}) {
+      phobic_tuning t = win;
+      t.search_waves = w;
+      t.search_mode = mode;
+      float ms;
+      const int r = time_stages(t, kStages - 1, kStages, &ms);  // partition_search
+      if (r != PHOBIC_OK) {
+        ctx->tuning = saved;
+        cleanup();
+        return r;
+      }
+      if (ms < search_ms) {
+        search_ms = ms;
+        win.search_waves = w;
+        win.search_mode = mode;
+      }
+    }
+  // the whole build with the winning shape
+  float win_ms;
+  {
+    const int r = time_stages(win, 0, kStages, &win_ms);
+    if (r != PHOBIC_OK) {
+      ctx->tuning = saved;
+      cleanup();
+      return r;
+    }
+  }
   ctx->tuning = win;
   cleanup();
   if (best) *best = win;

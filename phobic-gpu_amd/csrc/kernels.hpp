@@ -86,6 +86,13 @@ struct SearchArgs {
   // pilots + B*set_part_base[s], interleaved by that set's partition count
   uint32_t sets;
   const uint32_t* set_part_base;
+  uint32_t mode;  // mask of kSearchSinglePilot / kSearchNarrow
+};
+
+// Pilot-search variants (identical results, different speed).
+enum : uint32_t {
+  kSearchSinglePilot = 1,  // one pilot per hash round for every bucket
+  kSearchNarrow = 2,       // scan passes of 2048 offsets for every table size
 };
 
 struct EvalArgs {

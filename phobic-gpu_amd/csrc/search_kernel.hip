@@ -15,15 +15,19 @@
 //      (hash(l1, hash(l2, pilot)) >> 1) mod ps + d (mod ps) are pairwise
 //      distinct and free; they are then marked taken.
 //
-// Slot state lives in an extended bitmap E of L = 2*ps + 128 bits with
-// E[i] = taken[i mod ps], so the 64 candidate offsets [D, D+64) of a key at
-// initial position p are one unaligned 64-bit read of E at p + D.  A trial
-// first tests offsets 0..63 with one window per key and a DPP OR-reduction;
-// if none is free it runs one bit-parallel pass in which lane l owns
-// offsets [64 l, 64 l + 64) and ORs the windows of all keys.  Both return
-// the smallest free offset, which is the reference's answer.
+// Slot state lives in an extended bitmap E with E[i] = taken[i mod tm] for
+// i < 2*tm + 128, so the candidate offsets [D, D+w) of a key at initial
+// position p are one unaligned w-bit read of E at p + D.  A trial scans all
+// offsets bit-parallel: lane l owns offsets [32 l, 32 l + 32) of a pass of
+// 2048, reads the 32-bit window of every key (consecutive words across the
+// lanes, so no bank conflicts) and ORs them; the lowest free bit of the
+// lowest lane is the smallest valid offset, which is the reference's answer.
+// Large buckets in a sparse table first try offsets 0..63 with one 64-bit
+// window per key and a DPP OR-reduction, which nearly always succeeds there.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "device_hash.hpp"
 #include "kernels.hpp"
@@ -34,8 +38,27 @@ namespace {
 constexpr uint32_t kMaxBucket = 256;  // sort_sum.comp BINS, search.comp MAX_BUCKET_SIZE
 
 // LDS words of one partition for tables of at most tcap slots
+// Sub-windows per lane of a scan pass: lane l owns offsets
+// o0 + 2048 h + [32 l, 32 l + 32) for h < subwindows(tm).
+__host__ __device__ inline uint32_t subwindows(uint32_t tm) { return tm <= 2048 ? 1 : tm <= 4096 ? 2 : 4; }
+
+// E words: marks reach bit 2*tm + 128; a scan pass reads up to word
+// (2*tm - 2) / 32 + 64 * subwindows(tm).
+__host__ __device__ inline uint32_t e_words(uint32_t tcap) { return (2 * tcap) / 32 + 64 * subwindows(tcap) + 8; }
+
+// LDS of one partition, in 32-bit words (an even count, so that every
+// wave's rank table is 8-byte aligned):
+//   rt[2B]  processing rank r -> {bucket | size << 16, start in the slice}
+//   cp[B]   steps 1-3: keys per bucket (LDS atomics); steps 5-6: pilot per bucket
+//   U       steps 1-2: hist[256], bucket count per t = 256 - size
+//           steps 3-4: bpos[B], next free index of each bucket in the slice
+//           steps 5-6: E[e_words], coll[tcap / 32 + 1] (collision check)
+__host__ __device__ inline uint32_t union_words(uint32_t tcap, uint32_t B) {
+  const uint32_t u = e_words(tcap) + (tcap + 31) / 32 + 1;
+  return max(u, max(256u, B));
+}
 __host__ __device__ inline uint32_t lds_words(uint32_t tcap, uint32_t B) {
-  return ((2 * tcap + 128) / 32 + 4) + ((tcap + 31) / 32 + 1) + 5 * B + 256;
+  return (3 * B + union_words(tcap, B) + 1) & ~1u;
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
@@ -62,6 +85,13 @@ constexpr uint32_t kDupCheckPilot = 1024;
 
 // records loaded per lane per step of the partition prologue
 constexpr uint32_t kLoadBatch = 8;
+
+// smallest bucket that tries offsets 0..63 first (while the table is at most
+// half full); below it the full scan alone is cheaper
+constexpr uint32_t kWin0MinKeys = 6;
+
+// smallest table for which mod_f32 is exact (quotients below 2^22)
+constexpr uint32_t kF32ModMin = 512;
 
 template <int CTRL, int ROWS, bool BOUND>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
@@ -101,38 +131,101 @@ __device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t M, uint32_t d) 
   return (uint32_t)__umul64hi(M * a, (uint64_t)d);
 }
 
-// readlane of r[k >> 6] from lane k & 63 (k wave-uniform)
-__device__ __forceinline__ uint32_t pick4(const uint32_t (&r)[4], uint32_t k) {
-  const uint32_t c = k >> 6;
-  const uint32_t src = c == 0 ? r[0] : c == 1 ? r[1] : c == 2 ? r[2] : r[3];
-  return __builtin_amdgcn_readlane(src, k & 63);
+// a mod d for a < 2^31 and 512 <= d < 2^24 without 32-bit integer
+// multiplies (quarter rate): the f32 quotient estimate q has a relative
+// error below 3 * 2^-24, so for a / d < 2^22 it is off by at most one, and
+// a - q*d (exact as a 24x24-bit product) needs one correction either way.
+__device__ __forceinline__ uint32_t mod_f32(uint32_t a, float rd, uint32_t d) {
+  const uint32_t q = (uint32_t)(__uint2float_rn(a) * rd);
+  int32_t r = (int32_t)(a - __umul24(q, d));
+  r += r < 0 ? (int32_t)d : 0;
+  r -= r >= (int32_t)d ? (int32_t)d : 0;
+  return (uint32_t)r;
 }
 
 __device__ __forceinline__ uint64_t low_mask(uint32_t bits) {  // bits >= 64 -> all ones
   return bits >= 64 ? ~0ull : ((1ull << bits) - 1);
 }
 
+
+// Smallest offset d < dlimit at which the s positions held by lanes
+// [gbase, gbase + s) of pos (C chunks of 64 lanes) are all free, or false.
+// One pass covers 2048 * H offsets: lane l ORs the 32-bit windows of every
+// key at o0 + 2048 h + 32 l (consecutive words across the lanes, read four
+// keys at a time so that the LDS reads overlap).
+template <int C, int H>
+__device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32_t (&pos)[C], uint32_t gbase,
+                                             uint32_t s, uint32_t dlimit, uint32_t& dsel) {
+  constexpr uint32_t kStep = (C > 1 || H >= 4) ? 2 : 4;
+  const uint32_t lane = __lane_id();
+  for (uint32_t o0 = 0; o0 < dlimit; o0 += 2048 * H) {
+    uint32_t acc[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) acc[h] = 0;
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      if (64 * j < s) {
+        // the last key repeats when fewer than kStep are left (OR is idempotent)
+        const uint32_t kl = min(s - 64 * j, 64u) - 1;
+        for (uint32_t k = 0; k <= kl; k += kStep) {
+          uint32_t x[kStep], lo[kStep][H], hi[kStep][H];
+#pragma unroll
+          for (uint32_t i = 0; i < kStep; i++) {
+            x[i] = __builtin_amdgcn_readlane(pos[j], gbase + min(k + i, kl)) + o0;
+            const uint32_t* ew = Elane + (x[i] >> 5);
+#pragma unroll
+            for (int h = 0; h < H; h++) {
+              lo[i][h] = ew[64 * h];
+              hi[i][h] = ew[64 * h + 1];
+            }
+          }
+#pragma unroll
+          for (uint32_t i = 0; i < kStep; i++)
+#pragma unroll
+            for (int h = 0; h < H; h++) acc[h] |= __builtin_amdgcn_alignbit(hi[i][h], lo[i][h], x[i] & 31);
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+      const uint32_t off = o0 + 2048 * h + 32 * lane;
+      const uint32_t valid = off >= dlimit ? 0u : dlimit - off >= 32 ? ~0u : (1u << (dlimit - off)) - 1;
+      const uint32_t fr = ~acc[h] & valid;
+      const uint64_t any = __ballot(fr != 0);
+      if (any) {
+        const uint32_t l0 = (uint32_t)__builtin_ctzll(any);
+        dsel = o0 + 2048 * h + 32 * l0 + (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(fr, l0));
+        return true;
+      }
+    }
+  }
+  return false;
+}
 }  // namespace
 
 template <uint32_t kWavesPerGroup>
-__global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(SearchArgs a) {
+__global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_per_eu(8))) void k_partition_search(
+    SearchArgs a) {
   extern __shared__ __align__(16) uint32_t smem[];
   const uint32_t lane = __lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
+  // wave-uniform values are made visibly uniform so that they live in
+  // SGPRs and loops over them stay scalar
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t cap = a.cap, B = a.B, tcap = a.tcap;
-  const uint32_t EW = (2 * tcap + 128) / 32 + 4, CW = (tcap + 31) / 32 + 1;
-  uint32_t* E = smem + wave * lds_words(tcap, B);
-  uint32_t* coll = E + EW;
-  uint32_t* bcnt = coll + CW;   // key count per bucket
-  uint32_t* bcur = bcnt + B;    // placement cursor
-  uint32_t* bstart = bcur + B;  // start inside the partition
-  uint32_t* order = bstart + B; // bucket index by processing rank
-  uint32_t* pil = order + B;    // pilot per bucket index
-  uint32_t* hist = pil + B;     // 256 bins, t = 256 - size
+  const uint32_t EW = e_words(tcap), CW = (tcap + 31) / 32 + 1;
+  uint32_t* rt = smem + wave * lds_words(tcap, B);  // 2 words per rank
+  uint32_t* cp = rt + 2 * B;   // counts, then pilots, per bucket index
+  uint32_t* U = cp + B;
+  uint32_t* hist = U;          // steps 1-2
+  uint32_t* bpos = U;          // steps 3-4
+  uint32_t* E = U;             // steps 5-6
+  uint32_t* coll = U + EW;
+  uint32_t* bcnt = cp;
+  uint32_t* pil = cp;
 
   const uint32_t p = blockIdx.x * kWavesPerGroup + wave;
   if (p >= a.np) return;
-  const uint32_t ps = a.pcount[p];
+  const uint32_t ps = __builtin_amdgcn_readfirstlane(a.pcount[p]);
   const uint64_t base = (uint64_t)p * cap;
   // output: pilots[p + b*np] (one build) or the owning set's slice
   uint32_t* out_pilots = a.pilots;
@@ -155,13 +248,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
     for (uint32_t b = lane; b < B; b += 64) out_pilots[out_p + (uint64_t)b * out_np] = 0;
     return;
   }
-  for (uint32_t i = lane; i < EW; i += 64) E[i] = 0;
-  for (uint32_t i = lane; i < CW; i += 64) coll[i] = 0;
-  for (uint32_t i = lane; i < B; i += 64) {
-    bcnt[i] = 0;
-    bcur[i] = 0;
-    pil[i] = 0;
-  }
+  for (uint32_t i = lane; i < B; i += 64) bcnt[i] = 0;
   for (uint32_t i = lane; i < 256; i += 64) hist[i] = 0;
   lds_sync();
 
@@ -192,7 +279,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
   lds_sync();
 
   // 2a. bin starts: exclusive scan over t = 256 - size (largest first)
-  uint32_t m;
+  uint32_t m, r_big;  // non-empty buckets; those with more than 64 keys
   {
     uint32_t v[4], loc = 0;
 #pragma unroll
@@ -203,6 +290,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
     const uint32_t incl = wave_incl_scan(loc);
     uint32_t run = incl - loc;
     m = __builtin_amdgcn_readlane(incl, 63);
+    r_big = __builtin_amdgcn_readlane(run, (256 - 64) / 4);  // bins t < 192: sizes 65..256
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       hist[4 * lane + j] = run;
@@ -225,7 +313,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
       const uint32_t t = 256 - v;
       const uint32_t bv = hist[t];
       if (mine) {
-        order[bv + __popcll(same & lanemask_lt())] = (uint32_t)idx;
+        rt[2 * (bv + __popcll(same & lanemask_lt()))] = (uint32_t)idx;
         act = false;
       }
       lds_sync();
@@ -236,15 +324,20 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
   }
   lds_sync();
 
-  // 3. bucket start offsets
+  // 3. bucket start offsets: the rank table and each bucket's cursor
   {
     uint32_t carry = 0;
     for (uint32_t r0 = 0; r0 < m; r0 += 64) {
       const uint32_t r = r0 + lane;
-      const uint32_t b = r < m ? order[r] : 0;
+      const uint32_t b = r < m ? rt[2 * r] : 0;
       const uint32_t s = r < m ? bcnt[b] : 0;
       const uint32_t incl = wave_incl_scan(s);
-      if (r < m) bstart[b] = carry + incl - s;
+      if (r < m) {
+        const uint32_t st = carry + incl - s;
+        rt[2 * r] = b | s << 16;
+        rt[2 * r + 1] = st;
+        bpos[b] = st;
+      }
       carry += __builtin_amdgcn_readlane(incl, 63);
     }
   }
@@ -265,149 +358,196 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void k_partition_search(Search
     for (uint32_t j = 0; j < kLoadBatch; j++) {
       if (m[j] != 0xFFFFFFFFu) {
         const uint32_t b = m[j] & 0xFFFFu;
-        sorted[bstart[b] + atomicAdd(&bcur[b], 1u)] = v[j];
+        sorted[atomicAdd(&bpos[b], 1u)] = v[j];
       }
     }
   }
   wave_sync();  // the sorted slice is read back by other lanes
+  // the union region becomes E and coll, the count array the pilots
+  for (uint32_t i = lane; i < EW + CW; i += 64) U[i] = 0;
+  for (uint32_t i = lane; i < B; i += 64) pil[i] = 0;
+  lds_sync();
 
   // 5. pilot search over a table of tm slots (tm = ps unless alpha < 1)
   const uint32_t tm = table_size(ps, a.a_ppm);
   const uint64_t M = ~0ull / (tm ? tm : 1) + 1;
+  const bool f32mod = tm >= kF32ModMin;
+  const float rd = (float)(1.0 / (double)(tm ? tm : 1));
   const uint32_t L = 2 * tm + 128;  // valid bits of E
-  const uint32_t nblk = (tm + 63) / 64;
-  const uint64_t blk_mask = lane < nblk ? low_mask(tm - 64 * lane) : 0ull;
+  const uint32_t* Elane = E + lane;
   uint32_t err = 0;
 
-  uint64_t cur[4];
+  uint64_t cur[4] = {0, 0, 0, 0};
   uint32_t cs = 0, cst = 0, cb = 0;
   if (m) {
-    cb = order[0];
-    cs = bcnt[cb];
-    cst = bstart[cb];
+    const uint2 e = reinterpret_cast<const uint2*>(rt)[0];
+    cb = __builtin_amdgcn_readfirstlane(e.x) & 0xFFFFu;
+    cs = __builtin_amdgcn_readfirstlane(e.x) >> 16;
+    cst = __builtin_amdgcn_readfirstlane(e.y);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const uint32_t k = lane + 64 * j;
-      cur[j] = k < cs ? sorted[cst + k] : 0;
+      if (64 * j < cs && k < cs) cur[j] = sorted[cst + k];
     }
   }
-  for (uint32_t r = 0; r < m; r++) {
-    // prefetch the next bucket's keys
-    uint64_t nxt[4];
-    uint32_t ns = 0, nst = 0, nb = 0;
-    if (r + 1 < m) {
-      nb = order[r + 1];
-      ns = bcnt[nb];
-      nst = bstart[nb];
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t k = lane + 64 * j;
-      nxt[j] = k < ns ? sorted[nst + k] : 0;
-    }
+  uint32_t placed = 0, r = 0;
+  const bool multi = !(a.mode & kSearchSinglePilot);
 
-    const uint32_t s = cs;
-    uint32_t pos[4] = {0, 0, 0, 0};
-    uint32_t pilot = 0, dsel = 0;
-    for (;;) {
-      const uint64_t vbase = (uint64_t)tm * pilot;
-      if (vbase > 0x7FFFFFFFull) {
-        err = kErrPilotOverflow;
-        break;
+  // Buckets r < r_end whose keys fit in C registers per lane (C chunks of
+  // 64), scanned with H sub-windows per lane.  Buckets of at most 32 keys
+  // (C = 1) hash G = 64 / 2^lg consecutive pilots per round: lane group g
+  // of 2^lg lanes holds the positions for pilot p0 + g.
+  auto search_buckets = [&](auto chunks, auto subwin, uint32_t r_end) {
+    constexpr int C = decltype(chunks)::value;
+    constexpr int H = decltype(subwin)::value;
+    for (; r < r_end; r++) {
+      // prefetch the next bucket's keys
+      uint64_t nxt[4] = {0, 0, 0, 0};
+      uint32_t ns = 0, nst = 0, nb = 0;
+      if (r + 1 < m) {
+        const uint2 e = reinterpret_cast<const uint2*>(rt)[r + 1];
+        nb = __builtin_amdgcn_readfirstlane(e.x) & 0xFFFFu;
+        ns = __builtin_amdgcn_readfirstlane(e.x) >> 16;
+        nst = __builtin_amdgcn_readfirstlane(e.y);
       }
-      const uint32_t dlimit = (uint32_t)min((uint64_t)tm, 0x80000000ull - vbase);  // d < dlimit
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const uint32_t k = lane + 64 * j;
-        if (k < s) pos[j] = fastmod(mix32((uint32_t)cur[j], mix32((uint32_t)(cur[j] >> 32), pilot)) >> 1, M, tm);
+        if (64 * j < ns && k < ns) nxt[j] = sorted[nst + k];
       }
-      if (s > 1) {
-        bool c = false;
+
+      const uint32_t s = cs;
+      // the d < 64 window test pays off for large buckets in a sparse table
+      const bool win0 = s >= kWin0MinKeys && 2 * placed < tm;
+      uint32_t lg = 6;
+      if (C == 1 && multi && !win0 && s <= 32) lg = s <= 1 ? 0 : 32 - __builtin_clz(s - 1);
+      const uint32_t G = 64u >> lg;
+      const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
+      uint64_t key[C];
+      if (C == 1) {
+        key[0] = G > 1 ? __shfl(cur[0], (int)kk, 64) : cur[0];
+      } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const uint32_t k = lane + 64 * j;
-          if (k < s) {
-            const uint32_t msk = 1u << (pos[j] & 31);
-            c |= (atomicOr(&coll[pos[j] >> 5], msk) & msk) != 0;
+        for (int j = 0; j < C; j++) key[j] = cur[j];
+      }
+      uint32_t pos[C];
+      uint32_t p0 = 0, psel = 0, gsel = 0, dsel = 0;
+      bool dupchk = false;
+      for (;;) {  // rounds of G pilots
+#pragma unroll
+        for (int j = 0; j < C; j++) {
+          const uint32_t k = kk + 64 * j;
+          pos[j] = 0;
+          if (64 * j < s && k < s) {
+            const uint32_t h = mix32((uint32_t)key[j], mix32((uint32_t)(key[j] >> 32), p0 + g)) >> 1;
+            pos[j] = f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm);
           }
         }
-        lds_sync();
+        bool done = false;
+        for (uint32_t gi = 0; gi < G; gi++) {
+          const uint32_t pilot = p0 + gi;
+          const uint64_t vbase = (uint64_t)tm * pilot;
+          if (vbase > 0x7FFFFFFFull) {
+            err = kErrPilotOverflow;
+            done = true;
+            break;
+          }
+          const uint32_t dlimit = (uint32_t)min((uint64_t)tm, 0x80000000ull - vbase);  // d < dlimit
+          bool found = false;
+          if (win0) {  // G = 1: offsets 0..63, one window per key, OR over keys
+            uint64_t t0 = 0;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const uint32_t k = lane + 64 * j;
-          if (k < s) coll[pos[j] >> 5] = 0;
-        }
-        lds_sync();
-        if (__any(c)) {
-          if (pilot == kDupCheckPilot) {
-            bool dup = false;
-            for (uint32_t k2 = 0; k2 < s; k2++) {
-              const uint64_t kv = sorted[cst + k2];
+            for (int j = 0; j < C; j++) {
+              const uint32_t k = lane + 64 * j;
+              if (64 * j < s && k < s) t0 |= window64(E, pos[j]);
+            }
+            const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
+            const uint64_t free0 = ~taken0 & low_mask(dlimit);
+            if (free0) {
+              dsel = (uint32_t)__builtin_ctzll(free0);
+              found = true;
+            }
+          }
+          if (!found) found = scan_offsets<C, H>(Elane, pos, gi << lg, s, dlimit, dsel);
+          if (!found) continue;
+          if (s > 1) {  // the positions must be pairwise distinct
+            bool c = false;
 #pragma unroll
-              for (int j = 0; j < 4; j++) {
-                const uint32_t k = lane + 64 * j;
-                if (k < s && k != k2 && cur[j] == kv) dup = true;
+            for (int j = 0; j < C; j++) {
+              const uint32_t k = kk + 64 * j;
+              if (g == gi && 64 * j < s && k < s) {
+                const uint32_t msk = 1u << (pos[j] & 31);
+                c |= (atomicOr(&coll[pos[j] >> 5], msk) & msk) != 0;
               }
             }
-            if (__any(dup)) {
-              err = kErrDuplicateKeys;
-              break;
+            lds_sync();
+#pragma unroll
+            for (int j = 0; j < C; j++) {
+              const uint32_t k = kk + 64 * j;
+              if (g == gi && 64 * j < s && k < s) coll[pos[j] >> 5] = 0;
+            }
+            lds_sync();
+            if (__any(c)) {
+              if (pilot >= kDupCheckPilot && !dupchk) {
+                dupchk = true;
+                bool dup = false;
+                for (uint32_t k2 = 0; k2 < s; k2++) {
+                  const uint64_t kv = sorted[cst + k2];
+#pragma unroll
+                  for (int j = 0; j < C; j++) {
+                    const uint32_t k = lane + 64 * j;
+                    if (k < s && k != k2 && cur[j] == kv) dup = true;
+                  }
+                }
+                if (__any(dup)) {
+                  err = kErrDuplicateKeys;
+                  done = true;
+                  break;
+                }
+              }
+              continue;
             }
           }
-          pilot++;
-          continue;
-        }
-      }
-      // offsets 0..63: one window per key, OR over keys
-      uint64_t t0 = 0;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const uint32_t k = lane + 64 * j;
-        if (k < s) t0 |= window64(E, pos[j]);
-      }
-      const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
-      const uint64_t free0 = ~taken0 & low_mask(dlimit);
-      if (free0) {
-        dsel = (uint32_t)__builtin_ctzll(free0);
-        break;
-      }
-      if (dlimit > 64) {
-        // all offsets: lane l owns [64 l, 64 l + 64)
-        uint64_t acc = 0;
-        const uint32_t off = 64 * (lane < nblk ? lane : 0);
-        for (uint32_t k = 0; k < s; k++) acc |= window64(E, pick4(pos, k) + off);
-        const uint64_t lim = lane < nblk ? low_mask(dlimit > 64 * lane ? dlimit - 64 * lane : 0) : 0ull;
-        const uint64_t fr = ~acc & blk_mask & lim;
-        const uint64_t any = __ballot(fr != 0);
-        if (any) {
-          const uint32_t l0 = (uint32_t)__builtin_ctzll(any);
-          const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)fr, l0);
-          const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(fr >> 32), l0);
-          const uint64_t f = ((uint64_t)hi << 32) | lo;
-          dsel = 64 * l0 + (uint32_t)__builtin_ctzll(f);
+          psel = pilot;
+          gsel = gi;
+          done = true;
           break;
         }
+        if (done) break;
+        p0 += G;
       }
-      pilot++;
-    }
-    if (err) break;
-    // mark the bucket's slots in every copy of E
+      if (err) return;
+      // mark the bucket's slots in every copy of E
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t k = lane + 64 * j;
-      if (k < s) {
-        uint32_t q = pos[j] + dsel;
-        if (q >= tm) q -= tm;
-        for (uint32_t x = q; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
+      for (int j = 0; j < C; j++) {
+        const uint32_t k = kk + 64 * j;
+        if (g == gsel && 64 * j < s && k < s) {
+          uint32_t q = pos[j] + dsel;
+          if (q >= tm) q -= tm;
+          for (uint32_t x = q; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
+        }
       }
-    }
-    if (lane == 0) pil[cb] = dsel + tm * pilot;
-    lds_sync();
+      if (lane == 0) pil[cb] = dsel + tm * psel;
+      placed += s;
+      lds_sync();
 #pragma unroll
-    for (int j = 0; j < 4; j++) cur[j] = nxt[j];
-    cs = ns;
-    cst = nst;
-    cb = nb;
+      for (int j = 0; j < 4; j++) cur[j] = nxt[j];
+      cs = ns;
+      cst = nst;
+      cb = nb;
+    }
+  };
+  // processing order is size-descending: buckets above 64 keys come first.
+  // They are few and meet a sparse table, so they use 2048-offset passes,
+  // which keeps the kernel's register count (and so its occupancy) low.
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I4 = std::integral_constant<int, 4>;
+  search_buckets(I4{}, I1{}, min(r_big, m));
+  if (!err) {
+    if (tm <= 2048 || (a.mode & kSearchNarrow)) search_buckets(I1{}, I1{}, m);
+    else if (tm <= 4096) search_buckets(I1{}, I2{}, m);
+    else search_buckets(I1{}, I4{}, m);
   }
   if (err && lane == 0) atomicOr(a.status, 1u << err);
   lds_sync();

@@ -65,11 +65,12 @@ class Tuning(ctypes.Structure):
         ("search_waves", ctypes.c_uint32),
         ("route_per", ctypes.c_uint32),
         ("max_fanout", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("search_mode", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
-        return {"search_waves": self.search_waves, "route_per": self.route_per, "max_fanout": self.max_fanout}
+        return {"search_waves": self.search_waves, "route_per": self.route_per, "max_fanout": self.max_fanout,
+                "search_mode": self.search_mode}
 
 
 class StageTimes(ctypes.Structure):
@@ -332,8 +333,8 @@ class Context:
         self.lib.phobic_get_tuning(self.h, ctypes.byref(t))
         return t.as_dict()
 
-    def set_tuning(self, search_waves: int, route_per: int, max_fanout: int) -> None:
-        t = Tuning(search_waves, route_per, max_fanout, 0)
+    def set_tuning(self, search_waves: int, route_per: int, max_fanout: int, search_mode: int = 0) -> None:
+        t = Tuning(search_waves, route_per, max_fanout, search_mode)
         st = self.lib.phobic_set_tuning(self.h, ctypes.byref(t))
         if st:
             raise PhobicError(st, "set_tuning")

@@ -42,6 +42,8 @@ def test_c1_bit_exact_vs_golden(ctx):
     (500_000, 4.5, 2500, 0, 5),      # reference example.cpp configuration
     (200_000, 12.0, 1024, 7, 6),
     (5_000, 3.0, 64, 0, 7),
+    (300_000, 8.0, 5000, 0, 8),      # tables above 4096 slots: four sub-windows per lane
+    (200_000, 4.0, 9000, 0, 9),      # tables above 8192 slots: several scan passes
 ])
 def test_u64_bit_exact_vs_oracle(ctx, n, lam, P, seed, kseed):
     keys = po.splitmix64(kseed, n)
@@ -214,6 +216,7 @@ def test_device_bijection_100m(ctx):
     (400_000, 6.0, 2048, 0.99, 4),
     (300_000, 8.0, 2500, 0.95, 5),
     (100_000, 7.0, 1024, 0.5, 6),
+    (200_000, 7.0, 2500, 0.55, 7),     # alpha tables above 4096 slots
 ])
 def test_alpha_remap_bit_exact(ctx, n, lam, P, alpha, kseed):
     keys = po.splitmix64(kseed, n)
@@ -245,3 +248,31 @@ def test_alpha_device_eval_bijection(ctx):
     hip_rt.synchronize()
     assert int(ds.to_numpy(np.uint32)[0]) == 0
     assert int(bad.to_numpy(np.uint64)[0]) == 0
+
+
+@pytest.mark.parametrize("n,lam,P,alpha,kseed", [
+    (300_000, 8.0, 2500, 1.0, 11),
+    (300_000, 8.0, 5000, 1.0, 12),
+    (200_000, 4.0, 9000, 1.0, 13),
+    (200_000, 7.0, 2500, 0.55, 14),
+])
+def test_search_modes_bit_exact(ctx, n, lam, P, alpha, kseed):
+    """Every pilot-search variant (one or several pilots per hash round,
+    wide or 2048-offset scan passes) gives the oracle's pilots."""
+    keys = po.splitmix64(kseed, n)
+    cfg = ph.make_config(lam, P, 0, alpha)
+    hk = po.hash_u64(keys, 0)
+    if alpha < 1.0:
+        st, opil, ooff, orm, _ = po.build_keys_alpha(hk, lam, P, alpha)
+    else:
+        st, opil, ooff = po.build_keys(hk, lam, P)
+    assert st == 0
+    saved = ctx.tuning()
+    try:
+        for mode in (0, 1, 2, 3):
+            ctx.set_tuning(saved["search_waves"], saved["route_per"], saved["max_fanout"], mode)
+            r = ctx.build_u64(keys, cfg)
+            assert np.array_equal(r.pilots, opil), mode
+            assert np.array_equal(r.offsets, ooff), mode
+    finally:
+        ctx.set_tuning(**saved)
