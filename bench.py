@@ -46,6 +46,23 @@ def stage_bytes_per_key(name: str, lam: float, P: int) -> float:
     }.get(name, 0.0)
 
 
+def profiled_traffic(kernel: str, n: int, lam: float, P: int, alpha: float):
+    """HBM bytes per launch of `kernel` for this workload, as measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known
+    size) and recorded in profiles/*.md; None when no profile covers it."""
+    import glob
+
+    want = {"kernel": kernel, "n": str(n), "lambda": str(lam), "P": str(P), "alpha": str(float(alpha))}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            if not line.startswith("traffic_bytes_per_launch "):
+                continue
+            kv = dict(tok.split("=", 1) for tok in line.split()[1:])
+            if all(kv.get(k) == v for k, v in want.items()):
+                return float(kv["bytes"])
+    return None
+
+
 def cpu_baseline(n_sample: int, lam: float, P: int, kseed: int) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -382,7 +399,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(dom, n, lam, P, alpha),
             "bytes_per_key": round(bpk, 3),
         },
     }
