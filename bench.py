@@ -1,8 +1,10 @@
 """Benchmark: plain PHOBIC build throughput on one MI355X.
 
 One step = one full device build (hash -> partition -> bucket sort ->
-pilot search) of n distinct synthetic u64 keys that are already resident in
-HBM, producing pilots u32[np*B] and partition offsets u32[np+1] in HBM.
+pilot search -> slot remapping) of n distinct synthetic u64 keys that are
+already resident in HBM, producing pilots u32[np*B], partition offsets
+u32[np+1] and, for alpha < 1, the remap table in HBM.  The default workload
+is 1e9 keys, lambda 7.5, partition size 2048, alpha 0.99.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
 
@@ -63,14 +65,17 @@ def profiled_traffic(kernel: str, n: int, lam: float, P: int, alpha: float):
     return None
 
 
-def cpu_baseline(n_sample: int, lam: float, P: int, kseed: int) -> dict:
+def cpu_baseline(n_sample: int, lam: float, P: int, alpha: float, kseed: int) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
     keys = po.splitmix64(kseed, n_sample)
     t0 = time.perf_counter()
     hk = po.hash_u64(keys, 0)
-    st, _, _ = po.build_keys(hk, lam, P)
+    if alpha < 1.0:
+        st = po.build_keys_alpha(hk, lam, P, alpha)[0]
+    else:
+        st = po.build_keys(hk, lam, P)[0]
     dt = time.perf_counter() - t0
     if st != 0:
         raise RuntimeError(f"oracle baseline failed with status {st}")
@@ -81,7 +86,8 @@ def cpu_baseline(n_sample: int, lam: float, P: int, kseed: int) -> dict:
         "cores": threads,
         "kind": "port",
         "sample": f"{n_sample} keys (splitmix64 seed {kseed}) of the same configuration: oracle XXH3 hash + "
-                  f"build (bucketing, bucket sort, pilot search), OpenMP over partitions, {dt:.1f} s",
+                  f"build (bucketing, bucket sort, pilot search{', slot remapping' if alpha < 1.0 else ''}), "
+                  f"OpenMP over partitions, {dt:.1f} s",
     }
 
 
@@ -288,9 +294,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1_000_000_000)
     ap.add_argument("--lam", type=float, default=7.5)
     ap.add_argument("--P", type=int, default=2048)
-    ap.add_argument("--alpha", type=float, default=1.0)
+    ap.add_argument("--alpha", type=float, default=0.99)
     ap.add_argument("--key-seed", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=16_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=64_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--mode", choices=["plain", "batch", "serve", "copy"], default="plain")
@@ -300,6 +306,8 @@ def main() -> int:
                     help="sample keys for the launch-shape autotuner (0 = default shapes)")
     ap.add_argument("--search-mode", type=int, default=None, help="pilot-search variant mask (0..3)")
     ap.add_argument("--search-waves", type=int, default=None, help="partitions per search workgroup")
+    ap.add_argument("--route-per", type=int, default=None, help="records per thread of a routing tile (8 or 16)")
+    ap.add_argument("--max-fanout", type=int, default=None, help="bins per partitioning level (16..128)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build path runs on a single device; --gpus {args.gpus} is not "
@@ -321,12 +329,11 @@ def main() -> int:
     tuning = None
     if args.autotune:
         tuning, _ = ctx.autotune(cfg, args.autotune)
-    if args.search_mode is not None or args.search_waves is not None:
+    overrides = {"search_mode": args.search_mode, "search_waves": args.search_waves, "route_per": args.route_per,
+                 "max_fanout": args.max_fanout}
+    if any(v is not None for v in overrides.values()):
         t = ctx.tuning()
-        if args.search_mode is not None:
-            t["search_mode"] = args.search_mode
-        if args.search_waves is not None:
-            t["search_waves"] = args.search_waves
+        t.update({k: v for k, v in overrides.items() if v is not None})
         ctx.set_tuning(**t)
     stream = hip_rt.Stream()
     dk = hip_rt.DeviceBuffer(n * 8)
@@ -412,7 +419,7 @@ def main() -> int:
         out["bijection_violations"] = int(bad.to_numpy(np.uint64)[0])
         del out_pos, bm, bad
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, lam, P, args.key_seed)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, lam, P, alpha, args.key_seed)
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0

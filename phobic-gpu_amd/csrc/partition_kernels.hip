@@ -51,36 +51,34 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 }
 
 // Routes the block's tile (record k of thread t valid when valid[k]) to
-// child bins child_first + child[k] of capacity child_cap.
+// child bins child_first + child[k] of capacity child_cap.  Wave 0 scans
+// the tile's bin histogram and takes the bins' global ranges with one
+// atomic each; the scatter into LDS overlaps those atomics.
 template <uint32_t kPer>
 __device__ __forceinline__ void route_tile(TileLds<kPer>& s, const uint64_t (&lo)[kPer], const uint32_t (&meta)[kPer],
                                            const uint32_t (&child)[kPer], const bool (&valid)[kPer],
-                                           uint32_t fanout, uint64_t child_first, uint64_t child_cap,
+                                           uint64_t child_first, uint64_t child_cap,
                                            uint32_t* __restrict__ child_count, uint64_t* __restrict__ out_lo,
                                            uint32_t* __restrict__ out_meta, uint32_t* __restrict__ status) {
+  static_assert(kMaxFanout == 128, "wave 0 scans two bins per lane");
   const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < fanout; i += kThreads) s.hist[i] = 0;
+  for (uint32_t i = tid; i < kMaxFanout; i += kThreads) s.hist[i] = 0;
   __syncthreads();
   uint32_t rank[kPer];
 #pragma unroll
   for (uint32_t k = 0; k < kPer; k++)
     if (valid[k]) rank[k] = atomicAdd(&s.hist[child[k]], 1u);
   __syncthreads();
-  // block exclusive scan of the histogram (fanout <= kThreads)
-  const uint32_t v = tid < fanout ? s.hist[tid] : 0;
-  const uint32_t incl = wave_incl_scan(v);
-  if ((tid & 63) == 63) s.wsum[tid >> 6] = incl;
-  __syncthreads();
-  uint32_t before = 0, total = 0;
-#pragma unroll
-  for (uint32_t w = 0; w < kThreads / 64; w++) {
-    const uint32_t x = s.wsum[w];
-    if (w < (tid >> 6)) before += x;
-    total += x;
-  }
-  if (tid < fanout) {
-    s.lstart[tid] = before + incl - v;
-    s.gbase[tid] = v ? atomicAdd(&child_count[child_first + tid], v) : 0;
+  uint32_t g0 = 0, g1 = 0;
+  if (tid < 64) {  // bins 2 tid and 2 tid + 1 (empty beyond the fan-out)
+    const uint32_t v0 = s.hist[2 * tid], v1 = s.hist[2 * tid + 1];
+    const uint32_t pair = v0 + v1;
+    const uint32_t incl = wave_incl_scan(pair);
+    s.lstart[2 * tid] = incl - pair;
+    s.lstart[2 * tid + 1] = incl - pair + v0;
+    if (tid == 63) s.wsum[0] = incl;  // records in the tile
+    if (v0) g0 = atomicAdd(&child_count[child_first + 2 * tid], v0);
+    if (v1) g1 = atomicAdd(&child_count[child_first + 2 * tid + 1], v1);
   }
   __syncthreads();
 #pragma unroll
@@ -92,7 +90,12 @@ __device__ __forceinline__ void route_tile(TileLds<kPer>& s, const uint64_t (&lo
       s.child[d] = (uint8_t)child[k];
     }
   }
+  if (tid < 64) {
+    s.gbase[2 * tid] = g0;
+    s.gbase[2 * tid + 1] = g1;
+  }
   __syncthreads();
+  const uint32_t total = s.wsum[0];
   bool over = false;
   for (uint32_t d = tid; d < total; d += kThreads) {
     const uint32_t c = s.child[d];
@@ -134,7 +137,19 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
   for (uint32_t i = threadIdx.x; i < kFulcs; i += kThreads) fl[i] = a.fulcs[i];
   __syncthreads();
   const uint64_t ntiles = (a.n + kTile - 1) / kTile;
+  // u64 keys: the next tile's keys are loaded while this tile is routed
+  uint64_t kv[kPer];
+  auto load_u64 = [&](uint64_t t, uint64_t (&dst)[kPer]) {
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+      const uint64_t i = t * kTile + k * kThreads + threadIdx.x;
+      dst[k] = i < a.n ? a.src.u64[i] : 0;
+    }
+  };
+  if (KIND == kKindU64 && blockIdx.x < ntiles) load_u64(blockIdx.x, kv);
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    uint64_t nk[kPer];
+    if (KIND == kKindU64 && t + gridDim.x < ntiles) load_u64(t + gridDim.x, nk);
     uint64_t lo[kPer];
     uint32_t meta[kPer], child[kPer];
     bool valid[kPer];
@@ -157,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
       meta[k] = 0;
       child[k] = 0;
       if (valid[k]) {
-        const U128 h = hash_key<KIND>(a.src, i, a.seed);
+        const U128 h = KIND == kKindU64 ? xxh3_128_u64(kv[k], a.seed) : hash_key<KIND>(a.src, i, a.seed);
         uint32_t p;
         if (a.sets) {
           while (set + 1 < a.sets && a.set_key_off[set + 1] <= i) set++;
@@ -173,8 +188,11 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
         child[k] = c;
       }
     }
-    route_tile(s, lo, meta, child, valid, a.fanout, 0, (uint64_t)a.G * a.cap, a.child_count, a.out_lo, a.out_meta,
-               a.status);
+    route_tile(s, lo, meta, child, valid, 0, (uint64_t)a.G * a.cap, a.child_count, a.out_lo, a.out_meta, a.status);
+    if (KIND == kKindU64) {
+#pragma unroll
+      for (uint32_t k = 0; k < kPer; k++) kv[k] = nk[k];
+    }
   }
 }
 
@@ -208,8 +226,8 @@ __global__ __launch_bounds__(kThreads) void k_split_level(SplitArgs a) {
       child[k] = c;
     }
   }
-  route_tile(s, lo, meta, child, valid, a.fanout, (uint64_t)parent * a.fanout, (uint64_t)a.G_child * a.cap,
-             a.child_count, a.out_lo, a.out_meta, a.status);
+  route_tile(s, lo, meta, child, valid, (uint64_t)parent * a.fanout, (uint64_t)a.G_child * a.cap, a.child_count,
+             a.out_lo, a.out_meta, a.status);
 }
 
 uint32_t route_tile_records(uint32_t per) { return kThreads * (per == 8 ? 8 : 16); }

@@ -119,11 +119,11 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-// 64 bits of E starting at bit x.
+// 64 bits of E starting at the packed slot x.
 __device__ __forceinline__ uint64_t window64(const uint32_t* E, uint32_t x) {
-  const uint32_t w = x >> 5, sh = x & 31;
-  const uint32_t a = E[w], b = E[w + 1], c = E[w + 2];
-  return ((uint64_t)__builtin_amdgcn_alignbit(c, b, sh) << 32) | __builtin_amdgcn_alignbit(b, a, sh);
+  const uint32_t* ew = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(E) + (x >> 5));
+  const uint32_t a = ew[0], b = ew[1], c = ew[2];
+  return ((uint64_t)__builtin_amdgcn_alignbit(c, b, x) << 32) | __builtin_amdgcn_alignbit(b, a, x);
 }
 
 // Lemire's fastmod (M = floor((2^64-1)/d) + 1): a mod d for 32-bit a, d.
@@ -148,31 +148,47 @@ __device__ __forceinline__ uint64_t low_mask(uint32_t bits) {  // bits >= 64 -> 
 }
 
 
-// Smallest offset d < dlimit at which the s positions held by lanes
-// [gbase, gbase + s) of pos (C chunks of 64 lanes) are all free, or false.
-// One pass covers 2048 * H offsets: lane l ORs the 32-bit windows of every
-// key at o0 + 2048 h + 32 l (consecutive words across the lanes, read four
-// keys at a time so that the LDS reads overlap).
-template <int C, int H>
-__device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32_t (&pos)[C], uint32_t gbase,
-                                             uint32_t s, uint32_t dlimit, uint32_t& dsel) {
+// Packed slot q: (q >> 5) << 7 | (q & 31).  Shifted right by 5 it is the
+// byte offset of q's bitmap word; v_alignbit reads only its low 5 bits.
+__device__ __forceinline__ uint32_t pack_slot(uint32_t q) { return ((q & ~31u) << 2) | (q & 31); }
+__device__ __forceinline__ uint32_t unpack_slot(uint32_t x) { return ((x >> 2) & ~31u) | (x & 31); }
+
+// Mask of the offsets [off, off + 32) below lim.
+__device__ __forceinline__ uint32_t below_mask(uint32_t off, uint32_t lim) {
+  return off >= lim ? 0u : lim - off >= 32 ? ~0u : (1u << (lim - off)) - 1;
+}
+
+// Smallest offset d < dlimit at which the s slots held by lanes
+// [gbase, gbase + s) of pk (packed, C chunks of 64 lanes) are all free, or
+// false.  One pass covers 2048 * H offsets: lane l ORs the 32-bit windows
+// of every key at o0 + 2048 h + 32 l (consecutive words across the lanes,
+// read kStep keys at a time so that the LDS reads overlap).  With kPad the
+// lanes [gbase + s, gbase + round_up(s, 4)) repeat a slot of the bucket, so
+// the key loop needs no clamp; `vfull` is the first pass's offset mask for
+// dlimit = tm.
+template <int C, int H, bool kPad>
+__device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32_t (&pk)[C], uint32_t gbase,
+                                             uint32_t s, uint32_t dlimit, bool full, const uint32_t (&vfull)[H],
+                                             uint32_t& dsel) {
   constexpr uint32_t kStep = (C > 1 || H >= 4) ? 2 : 4;
   const uint32_t lane = __lane_id();
   for (uint32_t o0 = 0; o0 < dlimit; o0 += 2048 * H) {
+    const char* eb = reinterpret_cast<const char*>(Elane + (o0 >> 5));
     uint32_t acc[H];
 #pragma unroll
     for (int h = 0; h < H; h++) acc[h] = 0;
 #pragma unroll
     for (int j = 0; j < C; j++) {
       if (64 * j < s) {
-        // the last key repeats when fewer than kStep are left (OR is idempotent)
+        // without padding the last key repeats when fewer than kStep are
+        // left (OR is idempotent)
         const uint32_t kl = min(s - 64 * j, 64u) - 1;
         for (uint32_t k = 0; k <= kl; k += kStep) {
           uint32_t x[kStep], lo[kStep][H], hi[kStep][H];
 #pragma unroll
           for (uint32_t i = 0; i < kStep; i++) {
-            x[i] = __builtin_amdgcn_readlane(pos[j], gbase + min(k + i, kl)) + o0;
-            const uint32_t* ew = Elane + (x[i] >> 5);
+            x[i] = __builtin_amdgcn_readlane(pk[j], kPad ? gbase + k + i : gbase + min(k + i, kl));
+            const uint32_t* ew = reinterpret_cast<const uint32_t*>(eb + (x[i] >> 5));
 #pragma unroll
             for (int h = 0; h < H; h++) {
               lo[i][h] = ew[64 * h];
@@ -182,14 +198,13 @@ __device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32
 #pragma unroll
           for (uint32_t i = 0; i < kStep; i++)
 #pragma unroll
-            for (int h = 0; h < H; h++) acc[h] |= __builtin_amdgcn_alignbit(hi[i][h], lo[i][h], x[i] & 31);
+            for (int h = 0; h < H; h++) acc[h] |= __builtin_amdgcn_alignbit(hi[i][h], lo[i][h], x[i]);
         }
       }
     }
 #pragma unroll
     for (int h = 0; h < H; h++) {
-      const uint32_t off = o0 + 2048 * h + 32 * lane;
-      const uint32_t valid = off >= dlimit ? 0u : dlimit - off >= 32 ? ~0u : (1u << (dlimit - off)) - 1;
+      const uint32_t valid = (full && o0 == 0) ? vfull[h] : below_mask(o0 + 2048 * h + 32 * lane, dlimit);
       const uint32_t fr = ~acc[h] & valid;
       const uint64_t any = __ballot(fr != 0);
       if (any) {
@@ -374,6 +389,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   const bool f32mod = tm >= kF32ModMin;
   const float rd = (float)(1.0 / (double)(tm ? tm : 1));
   const uint32_t L = 2 * tm + 128;  // valid bits of E
+  // largest pilot whose offsets [0, tm) all keep V below 2^31
+  const uint32_t pilot_full = tm ? (uint32_t)(0x80000000ull / tm) - 1 : 0;
   const uint32_t* Elane = E + lane;
   uint32_t err = 0;
 
@@ -396,10 +413,15 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // Buckets r < r_end whose keys fit in C registers per lane (C chunks of
   // 64), scanned with H sub-windows per lane.  Buckets of at most 32 keys
   // (C = 1) hash G = 64 / 2^lg consecutive pilots per round: lane group g
-  // of 2^lg lanes holds the positions for pilot p0 + g.
+  // of 2^lg >= 4 lanes holds the slots for pilot p0 + g, its lanes past s
+  // repeating the last key.
   auto search_buckets = [&](auto chunks, auto subwin, uint32_t r_end) {
     constexpr int C = decltype(chunks)::value;
     constexpr int H = decltype(subwin)::value;
+    constexpr bool kPad = C == 1;
+    uint32_t vfull[H];  // first-pass offset masks for dlimit = tm
+#pragma unroll
+    for (int h = 0; h < H; h++) vfull[h] = below_mask(2048 * h + 32 * lane, tm);
     for (; r < r_end; r++) {
       // prefetch the next bucket's keys
       uint64_t nxt[4] = {0, 0, 0, 0};
@@ -420,46 +442,49 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       // the d < 64 window test pays off for large buckets in a sparse table
       const bool win0 = s >= kWin0MinKeys && 2 * placed < tm;
       uint32_t lg = 6;
-      if (C == 1 && multi && !win0 && s <= 32) lg = s <= 1 ? 0 : 32 - __builtin_clz(s - 1);
+      if (C == 1 && multi && !win0 && s <= 32) lg = s <= 4 ? 2 : 32 - __builtin_clz(s - 1);
       const uint32_t G = 64u >> lg;
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
       uint64_t key[C];
       if (C == 1) {
-        key[0] = G > 1 ? __shfl(cur[0], (int)kk, 64) : cur[0];
+        key[0] = __shfl(cur[0], (int)min(kk, s - 1), 64);
       } else {
 #pragma unroll
         for (int j = 0; j < C; j++) key[j] = cur[j];
       }
-      uint32_t pos[C];
+      uint32_t pk[C];  // packed slots
       uint32_t p0 = 0, psel = 0, gsel = 0, dsel = 0;
       bool dupchk = false;
       for (;;) {  // rounds of G pilots
 #pragma unroll
         for (int j = 0; j < C; j++) {
           const uint32_t k = kk + 64 * j;
-          pos[j] = 0;
-          if (64 * j < s && k < s) {
+          pk[j] = 0;
+          if (64 * j < s && (kPad || k < s)) {
             const uint32_t h = mix32((uint32_t)key[j], mix32((uint32_t)(key[j] >> 32), p0 + g)) >> 1;
-            pos[j] = f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm);
+            pk[j] = pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
           }
         }
         bool done = false;
         for (uint32_t gi = 0; gi < G; gi++) {
           const uint32_t pilot = p0 + gi;
-          const uint64_t vbase = (uint64_t)tm * pilot;
-          if (vbase > 0x7FFFFFFFull) {
-            err = kErrPilotOverflow;
-            done = true;
-            break;
+          uint32_t dlimit = tm;  // d < dlimit keeps V = d + tm * pilot below 2^31
+          if (pilot > pilot_full) {
+            const uint64_t vbase = (uint64_t)tm * pilot;
+            if (vbase > 0x7FFFFFFFull) {
+              err = kErrPilotOverflow;
+              done = true;
+              break;
+            }
+            dlimit = (uint32_t)(0x80000000ull - vbase);
           }
-          const uint32_t dlimit = (uint32_t)min((uint64_t)tm, 0x80000000ull - vbase);  // d < dlimit
           bool found = false;
           if (win0) {  // G = 1: offsets 0..63, one window per key, OR over keys
             uint64_t t0 = 0;
 #pragma unroll
             for (int j = 0; j < C; j++) {
               const uint32_t k = lane + 64 * j;
-              if (64 * j < s && k < s) t0 |= window64(E, pos[j]);
+              if (64 * j < s && (kPad || k < s)) t0 |= window64(E, pk[j]);
             }
             const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
             const uint64_t free0 = ~taken0 & low_mask(dlimit);
@@ -468,23 +493,23 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
               found = true;
             }
           }
-          if (!found) found = scan_offsets<C, H>(Elane, pos, gi << lg, s, dlimit, dsel);
+          if (!found) found = scan_offsets<C, H, kPad>(Elane, pk, gi << lg, s, dlimit, dlimit == tm, vfull, dsel);
           if (!found) continue;
-          if (s > 1) {  // the positions must be pairwise distinct
+          if (s > 1) {  // the slots must be pairwise distinct
             bool c = false;
 #pragma unroll
             for (int j = 0; j < C; j++) {
               const uint32_t k = kk + 64 * j;
               if (g == gi && 64 * j < s && k < s) {
-                const uint32_t msk = 1u << (pos[j] & 31);
-                c |= (atomicOr(&coll[pos[j] >> 5], msk) & msk) != 0;
+                const uint32_t msk = 1u << (pk[j] & 31);
+                c |= (atomicOr(&coll[pk[j] >> 7], msk) & msk) != 0;
               }
             }
             lds_sync();
 #pragma unroll
             for (int j = 0; j < C; j++) {
               const uint32_t k = kk + 64 * j;
-              if (g == gi && 64 * j < s && k < s) coll[pos[j] >> 5] = 0;
+              if (g == gi && 64 * j < s && k < s) coll[pk[j] >> 7] = 0;
             }
             lds_sync();
             if (__any(c)) {
@@ -522,7 +547,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       for (int j = 0; j < C; j++) {
         const uint32_t k = kk + 64 * j;
         if (g == gsel && 64 * j < s && k < s) {
-          uint32_t q = pos[j] + dsel;
+          uint32_t q = unpack_slot(pk[j]) + dsel;
           if (q >= tm) q -= tm;
           for (uint32_t x = q; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
         }
