@@ -172,6 +172,45 @@ __device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32
                                              uint32_t& dsel) {
   constexpr uint32_t kStep = (C > 1 || H >= 4) ? 2 : 4;
   const uint32_t lane = __lane_id();
+  if (full && dlimit <= 2048 * H) {  // one pass, offset masks known
+    uint32_t acc[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) acc[h] = 0;
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      if (64 * j < s) {
+        const uint32_t kl = min(s - 64 * j, 64u) - 1;
+        for (uint32_t k = 0; k <= kl; k += kStep) {
+          uint32_t x[kStep], lo[kStep][H], hi[kStep][H];
+#pragma unroll
+          for (uint32_t i = 0; i < kStep; i++) {
+            x[i] = __builtin_amdgcn_readlane(pk[j], kPad ? gbase + k + i : gbase + min(k + i, kl));
+            const uint32_t* ew = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(Elane) + (x[i] >> 5));
+#pragma unroll
+            for (int h = 0; h < H; h++) {
+              lo[i][h] = ew[64 * h];
+              hi[i][h] = ew[64 * h + 1];
+            }
+          }
+#pragma unroll
+          for (uint32_t i = 0; i < kStep; i++)
+#pragma unroll
+            for (int h = 0; h < H; h++) acc[h] |= __builtin_amdgcn_alignbit(hi[i][h], lo[i][h], x[i]);
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+      const uint32_t fr = ~acc[h] & vfull[h];
+      const uint64_t any = __ballot(fr != 0);
+      if (any) {
+        const uint32_t l0 = (uint32_t)__builtin_ctzll(any);
+        dsel = 2048 * h + 32 * l0 + (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(fr, l0));
+        return true;
+      }
+    }
+    return false;
+  }
   for (uint32_t o0 = 0; o0 < dlimit; o0 += 2048 * H) {
     const char* eb = reinterpret_cast<const char*>(Elane + (o0 >> 5));
     uint32_t acc[H];
@@ -432,8 +471,9 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         ns = __builtin_amdgcn_readfirstlane(e.x) >> 16;
         nst = __builtin_amdgcn_readfirstlane(e.y);
       }
+      // buckets after a C = 1 bucket have at most 64 keys too
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < C; j++) {
         const uint32_t k = lane + 64 * j;
         if (64 * j < ns && k < ns) nxt[j] = sorted[nst + k];
       }
@@ -549,14 +589,17 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         if (g == gsel && 64 * j < s && k < s) {
           uint32_t q = unpack_slot(pk[j]) + dsel;
           if (q >= tm) q -= tm;
-          for (uint32_t x = q; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
+          const uint32_t q1 = q + tm;  // both below L = 2 tm + 128
+          atomicOr(&E[q >> 5], 1u << (q & 31));
+          atomicOr(&E[q1 >> 5], 1u << (q1 & 31));
+          for (uint32_t x = q1 + tm; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
         }
       }
       if (lane == 0) pil[cb] = dsel + tm * psel;
       placed += s;
       lds_sync();
 #pragma unroll
-      for (int j = 0; j < 4; j++) cur[j] = nxt[j];
+      for (int j = 0; j < C; j++) cur[j] = nxt[j];
       cs = ns;
       cst = nst;
       cb = nb;
