@@ -452,7 +452,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // Buckets r < r_end whose keys fit in C registers per lane (C chunks of
   // 64), scanned with H sub-windows per lane.  Buckets of at most 32 keys
   // (C = 1) hash G = 64 / 2^lg consecutive pilots per round: lane group g
-  // of 2^lg >= 4 lanes holds the slots for pilot p0 + g, its lanes past s
+  // of 2^lg >= 4 lanes holds the slots for the round's pilot g, its lanes past s
   // repeating the last key.
   auto search_buckets = [&](auto chunks, auto subwin, uint32_t r_end) {
     constexpr int C = decltype(chunks)::value;
@@ -493,95 +493,87 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         for (int j = 0; j < C; j++) key[j] = cur[j];
       }
       uint32_t pk[C];  // packed slots
-      uint32_t p0 = 0, psel = 0, gsel = 0, dsel = 0;
+      uint32_t pilot = 0, dsel = 0;
       bool dupchk = false;
-      for (;;) {  // rounds of G pilots
+      for (;; pilot++) {
+        const uint32_t gi = pilot & (G - 1);
+        if (gi == 0) {  // slots for pilots pilot .. pilot + G - 1
 #pragma unroll
-        for (int j = 0; j < C; j++) {
-          const uint32_t k = kk + 64 * j;
-          pk[j] = 0;
-          if (64 * j < s && (kPad || k < s)) {
-            const uint32_t h = mix32((uint32_t)key[j], mix32((uint32_t)(key[j] >> 32), p0 + g)) >> 1;
-            pk[j] = pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
+          for (int j = 0; j < C; j++) {
+            const uint32_t k = kk + 64 * j;
+            pk[j] = 0;
+            if (64 * j < s && (kPad || k < s)) {
+              const uint32_t h = mix32((uint32_t)key[j], mix32((uint32_t)(key[j] >> 32), pilot + g)) >> 1;
+              pk[j] = pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
+            }
           }
         }
-        bool done = false;
-        for (uint32_t gi = 0; gi < G; gi++) {
-          const uint32_t pilot = p0 + gi;
-          uint32_t dlimit = tm;  // d < dlimit keeps V = d + tm * pilot below 2^31
-          if (pilot > pilot_full) {
-            const uint64_t vbase = (uint64_t)tm * pilot;
-            if (vbase > 0x7FFFFFFFull) {
-              err = kErrPilotOverflow;
-              done = true;
-              break;
-            }
-            dlimit = (uint32_t)(0x80000000ull - vbase);
+        uint32_t dlimit = tm;  // d < dlimit keeps V = d + tm * pilot below 2^31
+        if (pilot > pilot_full) {
+          const uint64_t vbase = (uint64_t)tm * pilot;
+          if (vbase > 0x7FFFFFFFull) {
+            err = kErrPilotOverflow;
+            break;
           }
-          bool found = false;
-          if (win0) {  // G = 1: offsets 0..63, one window per key, OR over keys
-            uint64_t t0 = 0;
-#pragma unroll
-            for (int j = 0; j < C; j++) {
-              const uint32_t k = lane + 64 * j;
-              if (64 * j < s && (kPad || k < s)) t0 |= window64(E, pk[j]);
-            }
-            const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
-            const uint64_t free0 = ~taken0 & low_mask(dlimit);
-            if (free0) {
-              dsel = (uint32_t)__builtin_ctzll(free0);
-              found = true;
-            }
-          }
-          if (!found) found = scan_offsets<C, H, kPad>(Elane, pk, gi << lg, s, dlimit, dlimit == tm, vfull, dsel);
-          if (!found) continue;
-          if (s > 1) {  // the slots must be pairwise distinct
-            bool c = false;
-#pragma unroll
-            for (int j = 0; j < C; j++) {
-              const uint32_t k = kk + 64 * j;
-              if (g == gi && 64 * j < s && k < s) {
-                const uint32_t msk = 1u << (pk[j] & 31);
-                c |= (atomicOr(&coll[pk[j] >> 7], msk) & msk) != 0;
-              }
-            }
-            lds_sync();
-#pragma unroll
-            for (int j = 0; j < C; j++) {
-              const uint32_t k = kk + 64 * j;
-              if (g == gi && 64 * j < s && k < s) coll[pk[j] >> 7] = 0;
-            }
-            lds_sync();
-            if (__any(c)) {
-              if (pilot >= kDupCheckPilot && !dupchk) {
-                dupchk = true;
-                bool dup = false;
-                for (uint32_t k2 = 0; k2 < s; k2++) {
-                  const uint64_t kv = sorted[cst + k2];
-#pragma unroll
-                  for (int j = 0; j < C; j++) {
-                    const uint32_t k = lane + 64 * j;
-                    if (k < s && k != k2 && cur[j] == kv) dup = true;
-                  }
-                }
-                if (__any(dup)) {
-                  err = kErrDuplicateKeys;
-                  done = true;
-                  break;
-                }
-              }
-              continue;
-            }
-          }
-          psel = pilot;
-          gsel = gi;
-          done = true;
-          break;
+          dlimit = (uint32_t)(0x80000000ull - vbase);
         }
-        if (done) break;
-        p0 += G;
+        bool found = false;
+        if (win0) {  // G = 1: offsets 0..63, one window per key, OR over keys
+          uint64_t t0 = 0;
+#pragma unroll
+          for (int j = 0; j < C; j++) {
+            const uint32_t k = lane + 64 * j;
+            if (64 * j < s && (kPad || k < s)) t0 |= window64(E, pk[j]);
+          }
+          const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
+          const uint64_t free0 = ~taken0 & low_mask(dlimit);
+          if (free0) {
+            dsel = (uint32_t)__builtin_ctzll(free0);
+            found = true;
+          }
+        }
+        if (!found && !scan_offsets<C, H, kPad>(Elane, pk, gi << lg, s, dlimit, dlimit == tm, vfull, dsel)) continue;
+        if (s > 1) {  // the slots must be pairwise distinct
+          bool c = false;
+#pragma unroll
+          for (int j = 0; j < C; j++) {
+            const uint32_t k = kk + 64 * j;
+            if (g == gi && 64 * j < s && k < s) {
+              const uint32_t msk = 1u << (pk[j] & 31);
+              c |= (atomicOr(&coll[pk[j] >> 7], msk) & msk) != 0;
+            }
+          }
+          lds_sync();
+#pragma unroll
+          for (int j = 0; j < C; j++) {
+            const uint32_t k = kk + 64 * j;
+            if (g == gi && 64 * j < s && k < s) coll[pk[j] >> 7] = 0;
+          }
+          lds_sync();
+          if (__any(c)) {
+            if (pilot >= kDupCheckPilot && !dupchk) {
+              dupchk = true;
+              bool dup = false;
+              for (uint32_t k2 = 0; k2 < s; k2++) {
+                const uint64_t kv = sorted[cst + k2];
+#pragma unroll
+                for (int j = 0; j < C; j++) {
+                  const uint32_t k = lane + 64 * j;
+                  if (k < s && k != k2 && cur[j] == kv) dup = true;
+                }
+              }
+              if (__any(dup)) {
+                err = kErrDuplicateKeys;
+                break;
+              }
+            }
+            continue;
+          }
+        }
+        break;
       }
       if (err) return;
+      const uint32_t gsel = pilot & (G - 1), psel = pilot;
       // mark the bucket's slots in every copy of E
 #pragma unroll
       for (int j = 0; j < C; j++) {
