@@ -23,24 +23,59 @@ using namespace phobic;
 
 namespace {
 
+// Device memory pools, one per device, created on first use.  The release
+// threshold keeps freed memory in the pool.
+std::mutex g_pool_mu;
+hipMemPool_t g_pools[64] = {};
+
+hipError_t pool_of_current(hipMemPool_t* out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  if (!g_pools[dev]) {
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    e = hipMemPoolCreate(&pool, &props);
+    if (e != hipSuccess) return e;
+    uint64_t keep = ~0ull;
+    e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    if (e != hipSuccess) {
+      (void)hipMemPoolDestroy(pool);
+      return e;
+    }
+    g_pools[dev] = pool;
+  }
+  *out = g_pools[dev];
+  return hipSuccess;
+}
+
+// A growable device buffer; allocation and release are ordered on the
+// stream that uses it.
 struct DevBuf {
   void* ptr = nullptr;
   size_t cap = 0;
-  hipError_t ensure(size_t bytes) {
+  hipError_t ensure(size_t bytes, hipStream_t s) {
     if (bytes <= cap && ptr) return hipSuccess;
     if (ptr) {
-      hipError_t e = hipFree(ptr);
+      hipError_t e = phobic::dev_free(ptr, s);
       if (e != hipSuccess) return e;
       ptr = nullptr;
       cap = 0;
     }
     const size_t want = bytes ? bytes : 256;
-    hipError_t e = hipMalloc(&ptr, want);
+    hipError_t e = phobic::dev_alloc(&ptr, want, s);
     if (e == hipSuccess) cap = want;
+    else ptr = nullptr;
     return e;
   }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
+  void release(hipStream_t s) {
+    if (ptr) (void)phobic::dev_free(ptr, s);
     ptr = nullptr;
     cap = 0;
   }
@@ -179,7 +214,7 @@ int upload_fulcrums(phobic_ctx* ctx, const phobic_config* cfg, hipStream_t s) {
   config_fulcrums(cfg, f);
   if (ctx->fulcs.ptr && std::memcmp(f, ctx->f_host, sizeof(f)) == 0) return PHOBIC_OK;
   std::memcpy(ctx->f_host, f, sizeof(f));
-  TRY_HIP(ctx->fulcs.ensure(kFulcsTable * 4));
+  TRY_HIP(ctx->fulcs.ensure(kFulcsTable * 4, s));
   TRY_HIP(hipMemcpyAsync(ctx->fulcs.ptr, ctx->f_host, kFulcsTable * 4, hipMemcpyHostToDevice, s));
   TRY_HIP(hipStreamSynchronize(s));
   return PHOBIC_OK;
@@ -238,11 +273,11 @@ int prepare_build(phobic_ctx* ctx, const phobic_config* cfg, const Plan& pl0, hi
   const Plan pl = tuned(ctx, pl0);
   TRY(upload_fulcrums(ctx, cfg, s));
   const Layout ly = layout_of(pl);
-  TRY_HIP(ctx->pcount.ensure(((uint64_t)pl.np + pl.fanout) * 4));
-  TRY_HIP(ctx->lvl_count.ensure((ly.cnt_total + 1) * 4));
+  TRY_HIP(ctx->pcount.ensure(((uint64_t)pl.np + pl.fanout) * 4, s));
+  TRY_HIP(ctx->lvl_count.ensure((ly.cnt_total + 1) * 4, s));
   for (int i = 0; i < 2; i++) {
-    TRY_HIP(ctx->rec_lo[i].ensure(ly.slots * 8));
-    TRY_HIP(ctx->rec_meta[i].ensure(ly.slots * 4));
+    TRY_HIP(ctx->rec_lo[i].ensure(ly.slots * 8, s));
+    TRY_HIP(ctx->rec_meta[i].ensure(ly.slots * 4, s));
   }
   return PHOBIC_OK;
 }
@@ -390,19 +425,19 @@ int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const v
   if (!outputs_ok(out, pl)) return PHOBIC_E_INVALID_CONFIG;
   TRY(set_device(ctx));
   hipStream_t s = ctx->stream;
-  TRY_HIP(ctx->in_a.ensure(a_bytes));
+  TRY_HIP(ctx->in_a.ensure(a_bytes, s));
   if (a_bytes) TRY_HIP(hipMemcpyAsync(ctx->in_a.ptr, a, a_bytes, hipMemcpyHostToDevice, s));
   if (b) {
-    TRY_HIP(ctx->in_b.ensure(b_bytes));
+    TRY_HIP(ctx->in_b.ensure(b_bytes, s));
     TRY_HIP(hipMemcpyAsync(ctx->in_b.ptr, b, b_bytes, hipMemcpyHostToDevice, s));
   }
   const uint64_t NB = (uint64_t)pl.np * pl.B;
   const bool remap = pl.a_ppm < 1000000u;
-  TRY_HIP(ctx->out_pilots.ensure(NB * 4 + 4));  // + status word
-  TRY_HIP(ctx->out_offsets.ensure(((uint64_t)pl.np + 1) * 4));
+  TRY_HIP(ctx->out_pilots.ensure(NB * 4 + 4, s));  // + status word
+  TRY_HIP(ctx->out_offsets.ensure(((uint64_t)pl.np + 1) * 4, s));
   if (remap) {
-    TRY_HIP(ctx->out_remap.ensure(remap_capacity(pl) * 4));
-    TRY_HIP(ctx->out_roff.ensure(((uint64_t)pl.np + 1) * 4));
+    TRY_HIP(ctx->out_remap.ensure(remap_capacity(pl) * 4, s));
+    TRY_HIP(ctx->out_roff.ensure(((uint64_t)pl.np + 1) * 4, s));
   }
   uint32_t* d_status = ctx->out_pilots.as<uint32_t>() + NB;
   KeySource src{};
@@ -442,6 +477,20 @@ int device_build(phobic_ctx* ctx, int kind, const KeySource& src, uint64_t n, co
 }  // namespace
 
 hipStream_t phobic::ctx_stream(phobic_ctx* ctx) { return ctx->stream; }
+
+hipError_t phobic::dev_alloc(void** p, size_t bytes, hipStream_t s) {
+  hipMemPool_t pool = nullptr;
+  const hipError_t e = pool_of_current(&pool);
+  if (e != hipSuccess) return e;
+  return hipMallocFromPoolAsync(p, bytes ? bytes : 256, pool, s);
+}
+
+hipError_t phobic::dev_free(void* p, hipStream_t s) { return p ? hipFreeAsync(p, s) : hipSuccess; }
+
+void phobic::dev_trim() {
+  hipMemPool_t pool = nullptr;
+  if (pool_of_current(&pool) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
+}
 
 // A batch of independent key sets built together (one configuration).
 struct phobic_batch {
@@ -539,16 +588,17 @@ int phobic_batch_create(phobic_ctx* ctx, const phobic_config* cfg, const uint64_
   int st = set_device(ctx);
   hipStream_t s = ctx->stream;
   auto fail = [&](int code) {
-    b->d_key_off.release();
-    b->d_part_base.release();
-    b->d_goff.release();
+    b->d_key_off.release(s);
+    b->d_part_base.release(s);
+    b->d_goff.release(s);
+    (void)hipStreamSynchronize(s);
     delete b;
     return code;
   };
   if (st) return fail(st);
-  hipError_t e = b->d_key_off.ensure((uint64_t)(sets + 1) * 8);
-  if (e == hipSuccess) e = b->d_part_base.ensure((uint64_t)(sets + 1) * 4);
-  if (e == hipSuccess) e = b->d_goff.ensure(((uint64_t)pl.np + 1) * 4);
+  hipError_t e = b->d_key_off.ensure((uint64_t)(sets + 1) * 8, s);
+  if (e == hipSuccess) e = b->d_part_base.ensure((uint64_t)(sets + 1) * 4, s);
+  if (e == hipSuccess) e = b->d_goff.ensure(((uint64_t)pl.np + 1) * 4, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_key_off.ptr, set_offsets, (uint64_t)(sets + 1) * 8, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_part_base.ptr, pb.data(), (uint64_t)(sets + 1) * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -618,9 +668,13 @@ int phobic_batch_destroy(phobic_batch* b) {
   if (!b) return PHOBIC_OK;
   (void)hipSetDevice(b->ctx->device);
   batch_drop_graph(b);
-  b->d_key_off.release();
-  b->d_part_base.release();
-  b->d_goff.release();
+  // the batch's last launch may still read its tables on another stream
+  (void)hipDeviceSynchronize();
+  const hipStream_t s = b->ctx->stream;
+  b->d_key_off.release(s);
+  b->d_part_base.release(s);
+  b->d_goff.release(s);
+  (void)hipStreamSynchronize(s);
   delete b;
   return PHOBIC_OK;
 }
@@ -649,10 +703,14 @@ int phobic_ctx_create(int device, phobic_ctx** out) {
 int phobic_ctx_destroy(phobic_ctx* ctx) {
   if (!ctx) return PHOBIC_OK;
   (void)hipSetDevice(ctx->device);
+  // device builds may have run on caller streams
+  (void)hipDeviceSynchronize();
+  for (DevBuf* b : {&ctx->pcount, &ctx->lvl_count, &ctx->rec_lo[0], &ctx->rec_lo[1], &ctx->rec_meta[0],
+                    &ctx->rec_meta[1], &ctx->fulcs, &ctx->status, &ctx->in_a, &ctx->in_b, &ctx->out_pilots,
+                    &ctx->out_offsets, &ctx->out_remap, &ctx->out_roff})
+    b->release(ctx->stream);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (DevBuf* b : {&ctx->pcount, &ctx->lvl_count, &ctx->rec_lo[0], &ctx->rec_lo[1], &ctx->rec_meta[0], &ctx->rec_meta[1], &ctx->fulcs, &ctx->status, &ctx->in_a, &ctx->in_b,
-                    &ctx->out_pilots, &ctx->out_offsets, &ctx->out_remap, &ctx->out_roff})
-    b->release();
+  phobic::dev_trim();
   for (int i = 0; i <= kStages; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -757,7 +815,7 @@ int phobic_get_tuning(phobic_ctx* ctx, phobic_tuning* t) {
 int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* t) {
   if (!ctx || !t) return PHOBIC_E_INVALID_CONFIG;
   const uint32_t w = t->search_waves, per = t->route_per, f = t->max_fanout;
-  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 16) || f < 16 || f > kMaxFanout ||
+  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 12 || per == 16) || f < 16 || f > kMaxFanout ||
       t->search_mode > 3)
     return PHOBIC_E_INVALID_CONFIG;
   ctx->tuning = *t;
@@ -776,14 +834,15 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   // sample keys and outputs, freed on return
   DevBuf keys, pil, off, rm, ro, st;
   auto cleanup = [&]() {
-    for (DevBuf* b : {&keys, &pil, &off, &rm, &ro, &st}) b->release();
+    for (DevBuf* b : {&keys, &pil, &off, &rm, &ro, &st}) b->release(s);
+    (void)hipStreamSynchronize(s);
   };
-  hipError_t e = keys.ensure(sample_keys * 8);
-  if (e == hipSuccess) e = pil.ensure((uint64_t)pl.np * pl.B * 4);
-  if (e == hipSuccess) e = off.ensure(((uint64_t)pl.np + 1) * 4);
-  if (e == hipSuccess) e = rm.ensure(rcap * 4 + 4);
-  if (e == hipSuccess) e = ro.ensure(((uint64_t)pl.np + 1) * 4);
-  if (e == hipSuccess) e = st.ensure(4);
+  hipError_t e = keys.ensure(sample_keys * 8, s);
+  if (e == hipSuccess) e = pil.ensure((uint64_t)pl.np * pl.B * 4, s);
+  if (e == hipSuccess) e = off.ensure(((uint64_t)pl.np + 1) * 4, s);
+  if (e == hipSuccess) e = rm.ensure(rcap * 4 + 4, s);
+  if (e == hipSuccess) e = ro.ensure(((uint64_t)pl.np + 1) * 4, s);
+  if (e == hipSuccess) e = st.ensure(4, s);
   if (e == hipSuccess) e = launch_splitmix64(0x5eed, sample_keys, keys.as<uint64_t>(), s);
   if (e != hipSuccess) {
     cleanup();
@@ -811,7 +870,7 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   };
   phobic_tuning win = saved;
   float part_ms = 1e30f, search_ms = 1e30f;
-  for (uint32_t per : {8u, 16u})
+  for (uint32_t per : {8u, 12u, 16u})
     for (uint32_t f : {32u, 64u, 128u}) {
       phobic_tuning t = win;
       t.route_per = per;

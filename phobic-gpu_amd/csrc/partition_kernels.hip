@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void k_split_level(SplitArgs a) {
              a.out_lo, a.out_meta, a.status);
 }
 
-uint32_t route_tile_records(uint32_t per) { return kThreads * (per == 8 ? 8 : 16); }
+uint32_t route_tile_records(uint32_t per) { return kThreads * (per == 8 ? 8 : per == 12 ? 12 : 16); }
 
 template <uint32_t kPer>
 static hipError_t hash_route_per(int kind, const HashRouteArgs& a, hipStream_t s) {
@@ -247,13 +247,17 @@ static hipError_t hash_route_per(int kind, const HashRouteArgs& a, hipStream_t s
 
 hipError_t launch_hash_route(int kind, const HashRouteArgs& a, uint32_t per, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
-  return per == 8 ? hash_route_per<8>(kind, a, s) : hash_route_per<16>(kind, a, s);
+  return per == 8 ? hash_route_per<8>(kind, a, s)
+       : per == 12 ? hash_route_per<12>(kind, a, s)
+                   : hash_route_per<16>(kind, a, s);
 }
 
 hipError_t launch_split_level(const SplitArgs& a, uint32_t parents, uint32_t per, hipStream_t s) {
   if (parents == 0) return hipSuccess;
   if (per == 8)
     hipLaunchKernelGGL(k_split_level<8>, dim3(parents * a.slices), dim3(kThreads), 0, s, a);
+  else if (per == 12)
+    hipLaunchKernelGGL(k_split_level<12>, dim3(parents * a.slices), dim3(kThreads), 0, s, a);
   else
     hipLaunchKernelGGL(k_split_level<16>, dim3(parents * a.slices), dim3(kThreads), 0, s, a);
   return hipGetLastError();

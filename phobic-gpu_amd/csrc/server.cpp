@@ -65,6 +65,7 @@ struct phobic_server {
   std::deque<std::shared_ptr<Request>> queue;
   std::map<uint64_t, std::shared_ptr<Request>> all;
   std::vector<Slot> slots;
+  hipStream_t free_stream = nullptr;  // releases of kept outputs
   uint64_t next_id = 1;
   uint64_t mem_in_use = 0;
   bool stop = false;
@@ -117,7 +118,7 @@ void try_admit(phobic_server* s) {
 int run_request(phobic_ctx* ctx, Request& r, hipStream_t stream) {
   const uint64_t n = r.req.n, NB = (uint64_t)r.np * r.B;
   // outputs kept on the device until released
-  if (hipMalloc(&r.d_out, r.keep_bytes) != hipSuccess) return PHOBIC_E_OOM;
+  if (phobic::dev_alloc(&r.d_out, r.keep_bytes, stream) != hipSuccess) return PHOBIC_E_OOM;
   uint8_t* p = static_cast<uint8_t*>(r.d_out);
   r.outs.pilots = reinterpret_cast<uint32_t*>(p);
   p += align256(NB * 4);
@@ -133,7 +134,7 @@ int run_request(phobic_ctx* ctx, Request& r, hipStream_t stream) {
   if (!r.req.keys_on_device) {  // stage host keys
     const uint64_t kb = r.req.kind == PHOBIC_KEYS_BYTES ? r.req.bytes_len : n * 8;
     const uint64_t ob = r.req.kind == PHOBIC_KEYS_BYTES ? (n + 1) * 8 : 0;
-    if (hipMalloc(&r.d_in, align256(kb) + ob + 256) != hipSuccess) return PHOBIC_E_OOM;
+    if (phobic::dev_alloc(&r.d_in, align256(kb) + ob + 256, stream) != hipSuccess) return PHOBIC_E_OOM;
     uint8_t* q = static_cast<uint8_t*>(r.d_in);
     if (kb && hipMemcpyAsync(q, r.req.keys, kb, hipMemcpyHostToDevice, stream) != hipSuccess) return PHOBIC_E_HIP;
     if (ob && hipMemcpyAsync(q + align256(kb), r.req.offsets, ob, hipMemcpyHostToDevice, stream) != hipSuccess)
@@ -152,7 +153,7 @@ int run_request(phobic_ctx* ctx, Request& r, hipStream_t stream) {
   if (hipMemcpyAsync(&bits, r.d_status, 4, hipMemcpyDeviceToHost, stream) != hipSuccess) return PHOBIC_E_HIP;
   if (hipStreamSynchronize(stream) != hipSuccess) return PHOBIC_E_HIP;
   if (r.d_in) {
-    (void)hipFree(r.d_in);
+    (void)phobic::dev_free(r.d_in, stream);
     r.d_in = nullptr;
   }
   for (int c = 1; c < 32; c++)
@@ -181,11 +182,11 @@ void worker(phobic_server* s, size_t idx, hipStream_t stream) {
       // scratch and staging returned; outputs stay until released
       s->mem_in_use -= job->est_bytes - (st == PHOBIC_OK ? job->keep_bytes : 0);
       if (st != PHOBIC_OK && job->d_out) {
-        (void)hipFree(job->d_out);
+        (void)phobic::dev_free(job->d_out, stream);
         job->d_out = nullptr;
       }
       if (job->d_in) {  // staging left behind by a failed request
-        (void)hipFree(job->d_in);
+        (void)phobic::dev_free(job->d_in, stream);
         job->d_in = nullptr;
       }
       s->completed++;
@@ -214,12 +215,18 @@ int phobic_server_create(const phobic_server_config* cfg, phobic_server** out) {
   if (!s) return PHOBIC_E_OOM;
   s->cfg = *cfg;
   if (s->cfg.queue_capacity == 0) s->cfg.queue_capacity = 1u << 16;
+  if (hipSetDevice(cfg->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->free_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return PHOBIC_E_NO_DEVICE;
+  }
   s->slots.resize(cfg->slots);
   for (auto& sl : s->slots) {
     const int st = phobic_ctx_create(cfg->device, &sl.ctx);
     if (st != PHOBIC_OK) {
       for (auto& x : s->slots)
         if (x.ctx) phobic_ctx_destroy(x.ctx);
+      (void)hipStreamDestroy(s->free_stream);
       delete s;
       return st;
     }
@@ -350,7 +357,7 @@ int phobic_server_release(phobic_server* s, uint64_t id) {
     if (r->d_out) s->mem_in_use -= r->keep_bytes;
   }
   (void)hipSetDevice(s->cfg.device);
-  if (r->d_out) (void)hipFree(r->d_out);
+  if (r->d_out) (void)phobic::dev_free(r->d_out, s->free_stream);
   r->d_out = nullptr;
   {
     std::lock_guard<std::mutex> lk(s->mu);
@@ -397,9 +404,11 @@ int phobic_server_destroy(phobic_server* s) {
     if (sl.th.joinable()) sl.th.join();
   (void)hipSetDevice(s->cfg.device);
   for (auto& kv : s->all) {
-    if (kv.second->d_out) (void)hipFree(kv.second->d_out);
-    if (kv.second->d_in) (void)hipFree(kv.second->d_in);
+    if (kv.second->d_out) (void)phobic::dev_free(kv.second->d_out, s->free_stream);
+    if (kv.second->d_in) (void)phobic::dev_free(kv.second->d_in, s->free_stream);
   }
+  (void)hipStreamSynchronize(s->free_stream);
+  (void)hipStreamDestroy(s->free_stream);
   for (auto& sl : s->slots)
     if (sl.ctx) phobic_ctx_destroy(sl.ctx);
   delete s;

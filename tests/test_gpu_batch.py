@@ -88,7 +88,7 @@ def test_tuning_does_not_change_results(ctx):
     saved = ctx.tuning()
     ref = ctx.build_u64(keys, cfg)
     try:
-        for w, per, f, mode in [(1, 8, 32, 1), (2, 16, 64, 2), (8, 8, 128, 3), (4, 16, 16, 0), (4, 16, 128, 3)]:
+        for w, per, f, mode in [(1, 8, 32, 1), (2, 16, 64, 2), (8, 8, 128, 3), (4, 16, 16, 0), (4, 16, 128, 3), (2, 12, 128, 0), (2, 12, 40, 1)]:
             ctx.set_tuning(w, per, f, mode)
             r = ctx.build_u64(keys, cfg)
             assert np.array_equal(r.pilots, ref.pilots) and np.array_equal(r.offsets, ref.offsets), (w, per, f, mode)
@@ -99,7 +99,7 @@ def test_tuning_does_not_change_results(ctx):
 def test_autotune_returns_valid_shape(ctx):
     saved = ctx.tuning()
     best, ms = ctx.autotune(ph.make_config(7.0, 2048), 2_000_000)
-    assert best["search_waves"] in (1, 2, 4, 8) and best["route_per"] in (8, 16)
+    assert best["search_waves"] in (1, 2, 4, 8) and best["route_per"] in (8, 12, 16)
     assert best["max_fanout"] in (32, 64, 128) and best["search_mode"] in (0, 1, 2, 3) and ms > 0
     assert ctx.tuning() == best
     ctx.set_tuning(**saved)
