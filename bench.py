@@ -308,6 +308,8 @@ def main() -> int:
     ap.add_argument("--search-waves", type=int, default=None, help="partitions per search workgroup")
     ap.add_argument("--route-per", type=int, default=None, help="records per thread of a routing tile (8 or 16)")
     ap.add_argument("--max-fanout", type=int, default=None, help="bins per partitioning level (16..128)")
+    ap.add_argument("--split-per", type=int, default=None, help="records per thread of a split tile (0 = route-per)")
+    ap.add_argument("--first-fanout", type=int, default=None, help="at most this many level-1 bins (0 = max-fanout)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build path runs on a single device; --gpus {args.gpus} is not "
@@ -330,7 +332,7 @@ def main() -> int:
     if args.autotune:
         tuning, _ = ctx.autotune(cfg, args.autotune)
     overrides = {"search_mode": args.search_mode, "search_waves": args.search_waves, "route_per": args.route_per,
-                 "max_fanout": args.max_fanout}
+                 "max_fanout": args.max_fanout, "split_per": args.split_per, "first_fanout": args.first_fanout}
     if any(v is not None for v in overrides.values()):
         t = ctx.tuning()
         t.update({k: v for k, v in overrides.items() if v is not None})

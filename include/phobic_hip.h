@@ -213,15 +213,21 @@ int phobic_server_destroy(phobic_server* server);
 
 /* ---- launch shapes and the autotuner ----
  * search_waves: partitions (one wave64 each) per search workgroup (1,2,4,8);
- * route_per: records per thread of a partitioning tile (8, 12 or 16);
+ * route_per: records per thread of a level-1 (hash and route) tile (8, 12
+ *   or 16);
  * max_fanout: bins per partitioning level (16..128);
  * search_mode: pilot-search variants, a mask of PHOBIC_SEARCH_* (0 = all
- * of the fast paths).  Defaults {2, 16, 128, 0}.
- * phobic_autotune times the partitioning shapes {8,12,16} x {32,64,128}
- * (on the partitioning stages' device time) and then the search shapes
- * {1,2,4,8} x {0,1,2,3} (on the search stage's time) on `sample_keys`
- * device-generated keys built with `cfg`, keeps the fastest on the context
- * and returns it.  Results do not depend on the tuning. */
+ *   of the fast paths);
+ * split_per: records per thread of a tile of levels 2..L (0 = route_per);
+ * first_fanout: at most this many level-1 bins (0 = max_fanout); the later
+ *   levels then take the larger share of the fan-out.
+ * Defaults {2, 16, 128, 0, 8, 32}.
+ * phobic_autotune times the partitioning shapes {8,12,16} x {8,16} x
+ * {0,16,32,64} x {64,128} (route_per x split_per x first_fanout x
+ * max_fanout, on the partitioning stages' device time) and then the search
+ * shapes {1,2,4,8} x {0,1,2,3} (on the search stage's time) on
+ * `sample_keys` device-generated keys built with `cfg`, keeps the fastest
+ * on the context and returns it.  Results do not depend on the tuning. */
 enum {
   PHOBIC_SEARCH_SINGLE_PILOT = 1, /* one pilot per hash round for every bucket */
   PHOBIC_SEARCH_NARROW = 2        /* scan passes of 2048 offsets for every table size */
@@ -231,6 +237,8 @@ typedef struct {
   uint32_t route_per;
   uint32_t max_fanout;
   uint32_t search_mode;
+  uint32_t split_per;
+  uint32_t first_fanout;
 } phobic_tuning;
 int phobic_get_tuning(phobic_ctx* ctx, phobic_tuning* tuning);
 int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* tuning);

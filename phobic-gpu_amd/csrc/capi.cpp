@@ -100,7 +100,7 @@ struct phobic_ctx {
   DevBuf in_a, in_b, out_pilots, out_offsets, out_remap, out_roff;
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
-  phobic_tuning tuning{2, 16, kMaxFanout, 0};
+  phobic_tuning tuning{2, 16, kMaxFanout, 0, 8, 32};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
@@ -131,31 +131,37 @@ struct Plan {
   uint64_t G[kMaxLevels + 1];
 };
 
-// Fan-out f <= max_fanout and the fewest levels L with f^L >= np; false
-// (plan untouched) when that needs more than kMaxLevels levels or a level-1
-// bin of more than 65535 partitions.
-bool plan_levels(Plan* pl, uint32_t max_fanout = kMaxFanout) {
+// Levels of the partitioning: the fewest L such that first * f^(L-1) >= np
+// with f <= max_fanout, where first (the level-1 bin count) is at most
+// first_fanout (0: max_fanout, all levels alike).  Level l's bins cover
+// G[l] = f^(L-l) partitions.  False (plan untouched) when that needs more
+// than kMaxLevels levels or a level-1 bin of more than 65535 partitions.
+bool plan_levels(Plan* pl, uint32_t max_fanout = kMaxFanout, uint32_t first_fanout = 0) {
   const uint32_t np = pl->np;
+  const uint32_t F1 = first_fanout ? std::min(first_fanout, max_fanout) : max_fanout;
+  auto power = [](uint64_t b, uint32_t e) {
+    uint64_t r = 1;
+    for (uint32_t i = 0; i < e; i++) r *= b;
+    return r;
+  };
   uint32_t L = 1, f = np ? np : 1;
-  if (np > max_fanout) {
+  if (np > F1) {
     L = 2;
-    while (std::pow((double)max_fanout, (double)L) < (double)np) L++;
+    while ((double)F1 * std::pow((double)max_fanout, (double)(L - 1)) < (double)np) L++;
     if (L > kMaxLevels) return false;
-    f = (uint32_t)std::ceil(std::pow((double)np, 1.0 / L));
-    auto reach = [&](uint32_t ff) {
-      uint64_t r = 1;
-      for (uint32_t i = 0; i < L; i++) r *= ff;
-      return r;
-    };
-    while (f > 2 && reach(f - 1) >= np) f--;
-    while (reach(f) < np) f++;
+    if (first_fanout) {  // smallest f with F1 * f^(L-1) >= np
+      f = (uint32_t)std::ceil(std::pow((double)np / F1, 1.0 / (L - 1)));
+      while (f > 2 && (uint64_t)F1 * power(f - 1, L - 1) >= np) f--;
+      while ((uint64_t)F1 * power(f, L - 1) < np) f++;
+    } else {  // smallest f with f^L >= np
+      f = (uint32_t)std::ceil(std::pow((double)np, 1.0 / L));
+      while (f > 2 && power(f - 1, L) >= np) f--;
+      while (power(f, L) < np) f++;
+    }
+    if (f > max_fanout) return false;
   }
   uint64_t G[kMaxLevels + 1] = {};
-  for (uint32_t l = 1; l <= L; l++) {
-    uint64_t g = 1;
-    for (uint32_t i = l; i < L; i++) g *= f;
-    G[l] = g;
-  }
+  for (uint32_t l = 1; l <= L; l++) G[l] = power(f, L - l);
   if (G[1] > 65535) return false;
   pl->L = L;
   pl->fanout = f;
@@ -255,7 +261,7 @@ Layout layout_of(const Plan& pl) {
 
 // The plan with the context's tuning applied (levels for its fan-out).
 Plan tuned(const phobic_ctx* ctx, Plan pl) {
-  if (!plan_levels(&pl, ctx->tuning.max_fanout)) plan_levels(&pl, kMaxFanout);  // validated plan kept
+  if (!plan_levels(&pl, ctx->tuning.max_fanout, ctx->tuning.first_fanout)) plan_levels(&pl, kMaxFanout);
   return pl;
 }
 
@@ -288,6 +294,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
                  const phobic_outputs& d_out, uint32_t* d_status, hipStream_t s, const BatchInfo& batch, bool timed) {
   const Plan pl = tuned(ctx, pl0);
   const uint32_t per = ctx->tuning.route_per;
+  const uint32_t split_per = ctx->tuning.split_per ? ctx->tuning.split_per : per;
   uint32_t* d_pilots = d_out.pilots;
   const bool remap = pl.a_ppm < 1000000u;
   const uint32_t np = pl.np, L = pl.L, f = pl.fanout;
@@ -329,7 +336,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
     sp.in_meta = ctx->rec_meta[out_buf(l - 1)].as<uint32_t>();
     sp.parent_count = counts(l - 1);
     sp.parent_cap = pl.G[l - 1] * pl.cap;
-    sp.slices = (uint32_t)((sp.parent_cap + route_tile_records(per) - 1) / route_tile_records(per));
+    sp.slices = (uint32_t)((sp.parent_cap + route_tile_records(split_per) - 1) / route_tile_records(split_per));
     sp.fanout = f;
     sp.G_child = (uint32_t)pl.G[l];
     sp.cap = pl.cap;
@@ -337,7 +344,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
     sp.out_lo = ctx->rec_lo[out_buf(l)].as<uint64_t>();
     sp.out_meta = ctx->rec_meta[out_buf(l)].as<uint32_t>();
     sp.status = d_status;
-    TRY_HIP(launch_split_level(sp, parents, per, s));
+    TRY_HIP(launch_split_level(sp, parents, split_per, s));
   }
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[2], s));
 
@@ -815,8 +822,9 @@ int phobic_get_tuning(phobic_ctx* ctx, phobic_tuning* t) {
 int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* t) {
   if (!ctx || !t) return PHOBIC_E_INVALID_CONFIG;
   const uint32_t w = t->search_waves, per = t->route_per, f = t->max_fanout;
+  const uint32_t sp = t->split_per, f1 = t->first_fanout;
   if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 12 || per == 16) || f < 16 || f > kMaxFanout ||
-      t->search_mode > 3)
+      t->search_mode > 3 || !(sp == 0 || sp == 8 || sp == 12 || sp == 16) || (f1 != 0 && (f1 < 2 || f1 > f)))
     return PHOBIC_E_INVALID_CONFIG;
   ctx->tuning = *t;
   return PHOBIC_OK;
@@ -871,23 +879,30 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   phobic_tuning win = saved;
   float part_ms = 1e30f, search_ms = 1e30f;
   for (uint32_t per : {8u, 12u, 16u})
-    for (uint32_t f : {32u, 64u, 128u}) {
-      phobic_tuning t = win;
-      t.route_per = per;
-      t.max_fanout = f;
-      float ms;
-      const int r = time_stages(t, 0, kStages - 2, &ms);  // hash_route + partition_split
-      if (r != PHOBIC_OK) {
-        ctx->tuning = saved;
-        cleanup();
-        return r;
-      }
-      if (ms < part_ms) {
-        part_ms = ms;
-        win.route_per = per;
-        win.max_fanout = f;
-      }
-    }
+    for (uint32_t sper : {8u, 16u})
+      for (uint32_t f1 : {0u, 16u, 32u, 64u})
+        for (uint32_t f : {64u, 128u}) {
+          if (f1 > f) continue;
+          phobic_tuning t = win;
+          t.route_per = per;
+          t.split_per = sper;
+          t.first_fanout = f1;
+          t.max_fanout = f;
+          float ms;
+          const int r = time_stages(t, 0, kStages - 2, &ms);  // hash_route + partition_split
+          if (r != PHOBIC_OK) {
+            ctx->tuning = saved;
+            cleanup();
+            return r;
+          }
+          if (ms < part_ms) {
+            part_ms = ms;
+            win.route_per = per;
+            win.split_per = sper;
+            win.first_fanout = f1;
+            win.max_fanout = f;
+          }
+        }
   for (uint32_t w : {1u, 2u, 4u, 8u})
     for (uint32_t mode : {0u, 1u, 2u, 3u}) {
       phobic_tuning t = win;

@@ -66,11 +66,13 @@ class Tuning(ctypes.Structure):
         ("route_per", ctypes.c_uint32),
         ("max_fanout", ctypes.c_uint32),
         ("search_mode", ctypes.c_uint32),
+        ("split_per", ctypes.c_uint32),
+        ("first_fanout", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
         return {"search_waves": self.search_waves, "route_per": self.route_per, "max_fanout": self.max_fanout,
-                "search_mode": self.search_mode}
+                "search_mode": self.search_mode, "split_per": self.split_per, "first_fanout": self.first_fanout}
 
 
 class StageTimes(ctypes.Structure):
@@ -333,8 +335,9 @@ class Context:
         self.lib.phobic_get_tuning(self.h, ctypes.byref(t))
         return t.as_dict()
 
-    def set_tuning(self, search_waves: int, route_per: int, max_fanout: int, search_mode: int = 0) -> None:
-        t = Tuning(search_waves, route_per, max_fanout, search_mode)
+    def set_tuning(self, search_waves: int, route_per: int, max_fanout: int, search_mode: int = 0,
+                   split_per: int = 0, first_fanout: int = 0) -> None:
+        t = Tuning(search_waves, route_per, max_fanout, search_mode, split_per, first_fanout)
         st = self.lib.phobic_set_tuning(self.h, ctypes.byref(t))
         if st:
             raise PhobicError(st, "set_tuning")

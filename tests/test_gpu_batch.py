@@ -88,10 +88,13 @@ def test_tuning_does_not_change_results(ctx):
     saved = ctx.tuning()
     ref = ctx.build_u64(keys, cfg)
     try:
-        for w, per, f, mode in [(1, 8, 32, 1), (2, 16, 64, 2), (8, 8, 128, 3), (4, 16, 16, 0), (4, 16, 128, 3), (2, 12, 128, 0), (2, 12, 40, 1)]:
-            ctx.set_tuning(w, per, f, mode)
+        shapes = [(1, 8, 32, 1, 0, 0), (2, 16, 64, 2, 0, 0), (8, 8, 128, 3, 0, 0), (4, 16, 16, 0, 0, 0),
+                  (4, 16, 128, 3, 0, 0), (2, 12, 128, 0, 0, 0), (2, 12, 40, 1, 0, 0), (2, 16, 128, 0, 8, 32),
+                  (2, 16, 64, 0, 16, 16), (1, 8, 128, 0, 12, 2), (2, 16, 16, 0, 8, 3)]
+        for shape in shapes:
+            ctx.set_tuning(*shape)
             r = ctx.build_u64(keys, cfg)
-            assert np.array_equal(r.pilots, ref.pilots) and np.array_equal(r.offsets, ref.offsets), (w, per, f, mode)
+            assert np.array_equal(r.pilots, ref.pilots) and np.array_equal(r.offsets, ref.offsets), shape
     finally:
         ctx.set_tuning(**saved)
 
@@ -100,6 +103,7 @@ def test_autotune_returns_valid_shape(ctx):
     saved = ctx.tuning()
     best, ms = ctx.autotune(ph.make_config(7.0, 2048), 2_000_000)
     assert best["search_waves"] in (1, 2, 4, 8) and best["route_per"] in (8, 12, 16)
-    assert best["max_fanout"] in (32, 64, 128) and best["search_mode"] in (0, 1, 2, 3) and ms > 0
+    assert best["max_fanout"] in (64, 128) and best["search_mode"] in (0, 1, 2, 3) and ms > 0
+    assert best["split_per"] in (8, 16) and best["first_fanout"] in (0, 16, 32, 64)
     assert ctx.tuning() == best
     ctx.set_tuning(**saved)
