@@ -8,8 +8,8 @@
 //                     few LDS-staged radix levels.  Replaces initial_hash
 //                     (mphf_builder.h:201-216), bucket_sizes.comp and
 //                     redistribute_keys.comp's global scatter.
-//   k_partition_offsets  exclusive scan of partition key counts
-//                     (prefix_sum.comp + prefix_sum_offset.comp).
+//   k_scan_totals/write  exclusive scans of partition key counts and remap
+//                     sizes (prefix_sum.comp + prefix_sum_offset.comp).
 //   k_partition_search   (search_kernel.hip) one wave64 per partition:
 //                     bucket counts, bucket sort (size desc, index desc),
 //                     redistribution and the pilot search (sort_sum.comp,
@@ -36,56 +36,103 @@ __device__ __forceinline__ void flag_error(uint32_t* status, uint32_t code) {
 }
 
 // ---------------------------------------------- partition offsets scan ----
-// One block of 1024 threads, 16 counts per thread per chunk, carried across
-// chunks.  offsets[0] = 0, offsets[p+1] = sum of counts[0..p].
+// Exclusive scans of the partition key counts (offsets, the reference's
+// prefix_sum.comp + prefix_sum_offset.comp with the host's leading 0,
+// mphf_builder.h:254-257) and, for alpha < 1, of the remap entries per
+// partition (table_size(count) - count).  Two launches over tiles of
+// kScanTile counts: tile totals, then every tile adds the totals before it
+// (at most 2^21 / kScanTile = 512 of them) and scans itself.
 
-__global__ __launch_bounds__(1024) void k_partition_offsets(const uint32_t* __restrict__ counts, uint32_t np,
-                                                            uint32_t a_ppm, uint32_t* __restrict__ offsets) {
-  __shared__ uint32_t wsum[16];
-  __shared__ uint32_t carry_s;
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  if (tid == 0) {
-    offsets[0] = 0;
-    carry_s = 0;
+constexpr uint32_t kScanThreads = 256, kScanPer = 16, kScanTile = kScanThreads * kScanPer;
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* wred) {
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (uint32_t o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if (lane == 0) wred[wid] = v;
+  __syncthreads();
+  uint32_t t = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kScanThreads / 64; w++) t += wred[w];
+  return t;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_totals(const uint32_t* __restrict__ counts, uint32_t np,
+                                                             uint32_t a_ppm, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t wred[kScanThreads / 64];
+  uint32_t c_sum = 0, r_sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < kScanPer; j++) {
+    const uint64_t idx = (uint64_t)blockIdx.x * kScanTile + j * kScanThreads + threadIdx.x;
+    const uint32_t c = idx < np ? counts[idx] : 0;
+    c_sum += c;
+    if (a_ppm) r_sum += table_size(c, a_ppm) - c;
+  }
+  c_sum = block_sum(c_sum, wred);
+  if (a_ppm) r_sum = block_sum(r_sum, wred);
+  if (threadIdx.x == 0) {
+    totals[2 * blockIdx.x] = c_sum;
+    totals[2 * blockIdx.x + 1] = r_sum;
+  }
+}
+
+// Exclusive block scan of one value per thread; returns the thread's prefix.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wred) {
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
   }
   __syncthreads();
-  constexpr uint32_t kPer = 16, kChunk = 1024 * kPer;
-  for (uint64_t c0 = 0; c0 < np; c0 += kChunk) {
-    uint32_t v[kPer];
-    uint32_t local = 0;
+  if (lane == 63) wred[wid] = incl;
+  __syncthreads();
+  uint32_t before = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < kPer; j++) {
-      const uint64_t idx = c0 + (uint64_t)tid * kPer + j;
-      const uint32_t c = idx < np ? counts[idx] : 0;
-      v[j] = a_ppm ? table_size(c, a_ppm) - c : c;
-      local += v[j];
-    }
-    // wave inclusive scan of per-thread totals
-    uint32_t incl = local;
+  for (uint32_t w = 0; w < kScanThreads / 64; w++)
+    if (w < wid) before += wred[w];
+  return before + incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_write(const uint32_t* __restrict__ counts, uint32_t np,
+                                                            uint32_t a_ppm, const uint32_t* __restrict__ totals,
+                                                            uint32_t* __restrict__ offsets,
+                                                            uint32_t* __restrict__ remap_offsets) {
+  __shared__ uint32_t wred[kScanThreads / 64];
+  // totals of the tiles before this one
+  uint32_t c_base = 0, r_base = 0;
+  for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanThreads) {
+    c_base += totals[2 * t];
+    r_base += totals[2 * t + 1];
+  }
+  c_base = block_sum(c_base, wred);
+  if (a_ppm) r_base = block_sum(r_base, wred);
+  // thread t scans counts [tile0 + t*kScanPer, +kScanPer)
+  const uint64_t i0 = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanPer;
+  uint32_t c[kScanPer];
+  uint32_t c_loc = 0, r_loc = 0;
 #pragma unroll
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-      const uint32_t t = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += t;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    uint32_t wbase = 0, chunk_total = 0;
-    for (uint32_t w = 0; w < 16; w++) {
-      const uint32_t ws = wsum[w];
-      if (w < wid) wbase += ws;
-      chunk_total += ws;
-    }
-    const uint32_t carry = carry_s;
-    uint32_t run = carry + wbase + incl - local;
+  for (uint32_t j = 0; j < kScanPer; j++) {
+    c[j] = i0 + j < np ? counts[i0 + j] : 0;
+    c_loc += c[j];
+    if (a_ppm) r_loc += table_size(c[j], a_ppm) - c[j];
+  }
+  uint32_t c_run = c_base + block_excl_scan(c_loc, wred);
+  uint32_t r_run = a_ppm ? r_base + block_excl_scan(r_loc, wred) : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    offsets[0] = 0;
+    if (a_ppm) remap_offsets[0] = 0;
+  }
 #pragma unroll
-    for (uint32_t j = 0; j < kPer; j++) {
-      const uint64_t idx = c0 + (uint64_t)tid * kPer + j;
-      run += v[j];
-      if (idx < np) offsets[idx + 1] = run;
+  for (uint32_t j = 0; j < kScanPer; j++) {
+    c_run += c[j];
+    if (a_ppm) r_run += table_size(c[j], a_ppm) - c[j];
+    if (i0 + j < np) {
+      offsets[i0 + j + 1] = c_run;
+      if (a_ppm) remap_offsets[i0 + j + 1] = r_run;
     }
-    __syncthreads();
-    if (tid == 0) carry_s = carry + chunk_total;
-    __syncthreads();
   }
 }
 
@@ -177,9 +224,19 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
+uint32_t partition_scan_tiles(uint32_t np) { return (np + kScanTile - 1) / kScanTile; }
+
 hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_t a_ppm, uint32_t* offsets,
-                                    hipStream_t s) {
-  hipLaunchKernelGGL(k_partition_offsets, dim3(1), dim3(1024), 0, s, counts, np, a_ppm, offsets);
+                                    uint32_t* remap_offsets, uint32_t* totals, hipStream_t s) {
+  const uint32_t tiles = partition_scan_tiles(np);
+  if (tiles == 0) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, offsets, 1ull, 0u);
+    if (a_ppm) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, remap_offsets, 1ull, 0u);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_scan_totals, dim3(tiles), dim3(kScanThreads), 0, s, counts, np, a_ppm, totals);
+  hipLaunchKernelGGL(k_scan_write, dim3(tiles), dim3(kScanThreads), 0, s, counts, np, a_ppm, totals, offsets,
+                     remap_offsets);
   return hipGetLastError();
 }
 

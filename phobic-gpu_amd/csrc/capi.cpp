@@ -86,6 +86,7 @@ struct DevBuf {
 };
 
 constexpr int kStages = 4;
+constexpr int kGraphScratch = 8;  // context buffers a captured build reads or writes
 const char* const kStageNames[kStages] = {"hash_route", "partition_split", "partition_offsets", "partition_search"};
 constexpr uint32_t kMaxLevels = 4;
 
@@ -95,7 +96,7 @@ struct phobic_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   // scratch owned by the context
-  DevBuf pcount, lvl_count, rec_lo[2], rec_meta[2], fulcs, status;
+  DevBuf pcount, lvl_count, scan_tot, rec_lo[2], rec_meta[2], fulcs, status;
   // staging for host-buffer entry points
   DevBuf in_a, in_b, out_pilots, out_offsets, out_remap, out_roff;
   hipEvent_t ev[kStages + 1] = {};
@@ -281,6 +282,7 @@ int prepare_build(phobic_ctx* ctx, const phobic_config* cfg, const Plan& pl0, hi
   const Layout ly = layout_of(pl);
   TRY_HIP(ctx->pcount.ensure(((uint64_t)pl.np + pl.fanout) * 4, s));
   TRY_HIP(ctx->lvl_count.ensure((ly.cnt_total + 1) * 4, s));
+  TRY_HIP(ctx->scan_tot.ensure((2ull * partition_scan_tiles(pl.np) + 2) * 4, s));
   for (int i = 0; i < 2; i++) {
     TRY_HIP(ctx->rec_lo[i].ensure(ly.slots * 8, s));
     TRY_HIP(ctx->rec_meta[i].ensure(ly.slots * 4, s));
@@ -348,14 +350,15 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
   }
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[2], s));
 
-  if (batch.sets) {
-    TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, 0, batch.global_offsets, s));
+  uint32_t* totals = ctx->scan_tot.as<uint32_t>();
+  if (batch.sets) {  // batches build at alpha = 1
+    TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, 0, batch.global_offsets, nullptr, totals, s));
     TRY_HIP(launch_batch_offsets(batch.global_offsets, batch.set_part_base, batch.sets, np + batch.sets,
                                  d_out.offsets, s));
   } else {
-    TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, 0, d_out.offsets, s));
+    TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, remap ? pl.a_ppm : 0, d_out.offsets,
+                                     remap ? d_out.remap_offsets : nullptr, totals, s));
   }
-  if (remap) TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, pl.a_ppm, d_out.remap_offsets, s));
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[3], s));
 
   SearchArgs ra{};
@@ -515,16 +518,17 @@ struct phobic_batch {
   const void* g_pilots = nullptr;
   const void* g_offsets = nullptr;
   const void* g_status = nullptr;
-  const void* g_scratch[7] = {};  // context scratch the graph's kernels use
+  const void* g_scratch[kGraphScratch] = {};  // context scratch the graph's kernels use
 };
 
 namespace {
 
 // Context scratch addresses baked into a captured graph.
-void scratch_ptrs(const phobic_ctx* ctx, const void* (&p)[7]) {
-  const void* v[7] = {ctx->pcount.ptr, ctx->lvl_count.ptr, ctx->rec_lo[0].ptr, ctx->rec_lo[1].ptr,
-                      ctx->rec_meta[0].ptr, ctx->rec_meta[1].ptr, ctx->fulcs.ptr};
-  for (int i = 0; i < 7; i++) p[i] = v[i];
+void scratch_ptrs(const phobic_ctx* ctx, const void* (&p)[kGraphScratch]) {
+  const void* v[kGraphScratch] = {ctx->pcount.ptr,    ctx->lvl_count.ptr,   ctx->scan_tot.ptr,
+                                  ctx->rec_lo[0].ptr, ctx->rec_lo[1].ptr,   ctx->rec_meta[0].ptr,
+                                  ctx->rec_meta[1].ptr, ctx->fulcs.ptr};
+  for (int i = 0; i < kGraphScratch; i++) p[i] = v[i];
 }
 
 void batch_drop_graph(phobic_batch* b) {
@@ -640,11 +644,11 @@ int phobic_batch_graph_launch(phobic_batch* b, const uint64_t* d_keys, uint32_t*
   // fulcrums re-uploaded and scratch grown as needed; a replay is valid
   // only while every address the graph captured is unchanged
   TRY(prepare_build(b->ctx, &b->cfg, b->pl, s));
-  const void* cur[7];
+  const void* cur[kGraphScratch];
   scratch_ptrs(b->ctx, cur);
   bool same = b->exec && b->g_keys == d_keys && b->g_pilots == d_pilots && b->g_offsets == d_offsets &&
               b->g_status == d_status;
-  for (int i = 0; i < 7 && same; i++) same = cur[i] == b->g_scratch[i];
+  for (int i = 0; i < kGraphScratch && same; i++) same = cur[i] == b->g_scratch[i];
   if (!same) {
     batch_drop_graph(b);
     hipStream_t cs = nullptr;
@@ -665,7 +669,7 @@ int phobic_batch_graph_launch(phobic_batch* b, const uint64_t* d_keys, uint32_t*
     b->g_pilots = d_pilots;
     b->g_offsets = d_offsets;
     b->g_status = d_status;
-    for (int i = 0; i < 7; i++) b->g_scratch[i] = cur[i];
+    for (int i = 0; i < kGraphScratch; i++) b->g_scratch[i] = cur[i];
   }
   TRY_HIP(hipGraphLaunch(b->exec, s));
   return PHOBIC_OK;
@@ -712,7 +716,7 @@ int phobic_ctx_destroy(phobic_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   // device builds may have run on caller streams
   (void)hipDeviceSynchronize();
-  for (DevBuf* b : {&ctx->pcount, &ctx->lvl_count, &ctx->rec_lo[0], &ctx->rec_lo[1], &ctx->rec_meta[0],
+  for (DevBuf* b : {&ctx->pcount, &ctx->lvl_count, &ctx->scan_tot, &ctx->rec_lo[0], &ctx->rec_lo[1], &ctx->rec_meta[0],
                     &ctx->rec_meta[1], &ctx->fulcs, &ctx->status, &ctx->in_a, &ctx->in_b, &ctx->out_pilots,
                     &ctx->out_offsets, &ctx->out_remap, &ctx->out_roff})
     b->release(ctx->stream);

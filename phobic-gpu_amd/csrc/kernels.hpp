@@ -114,9 +114,12 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_hash_route(int kind, const HashRouteArgs& a, uint32_t per, hipStream_t s);
 hipError_t launch_split_level(const SplitArgs& a, uint32_t parents, uint32_t per, hipStream_t s);
 uint32_t route_tile_records(uint32_t per);
-// exclusive scan of counts (a_ppm = 0) or of table_size(count) - count
+// exclusive scans of counts into offsets[np+1] and, when a_ppm > 0, of
+// table_size(count) - count into remap_offsets[np+1]; `totals` holds
+// 2 * partition_scan_tiles(np) words of scratch
+uint32_t partition_scan_tiles(uint32_t np);
 hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_t a_ppm, uint32_t* offsets,
-                                    hipStream_t s);
+                                    uint32_t* remap_offsets, uint32_t* totals, hipStream_t s);
 // waves = partitions (wave64 each) per workgroup: 1, 2, 4 or 8
 size_t search_lds_bytes(uint32_t tcap, uint32_t B, uint32_t waves);
 hipError_t launch_partition_search(const SearchArgs& a, uint32_t waves, hipStream_t s);

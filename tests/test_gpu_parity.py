@@ -276,3 +276,27 @@ def test_search_modes_bit_exact(ctx, n, lam, P, alpha, kseed):
             assert np.array_equal(r.offsets, ooff), mode
     finally:
         ctx.set_tuning(**saved)
+
+
+@pytest.mark.parametrize("alpha", [1.0, 0.97])
+def test_offsets_span_several_scan_tiles(ctx, alpha):
+    """20M keys at P = 2048 give 9766 partitions, three tiles of the offsets
+    scan: offsets are the exclusive scan of the partition counts and remap
+    offsets that of table_size(count) - count."""
+    n, lam, P = 20_000_000, 7.5, 2048
+    keys = po.splitmix64(31, n)
+    r = ctx.build_u64(keys, ph.make_config(lam, P, 0, alpha))
+    hk = po.hash_u64(keys, 0)
+    np_ = (n + P - 1) // P
+    part = (hk[:, 0].astype(np.uint64) * np.uint64(np_)) >> np.uint64(32)
+    cnt = np.bincount(part.astype(np.int64), minlength=np_)
+    off = np.zeros(np_ + 1, np.int64)
+    off[1:] = np.cumsum(cnt)
+    assert np.array_equal(r.offsets.astype(np.int64), off)
+    if alpha < 1.0:
+        a = po.alpha_ppm(alpha)
+        lib = po.load()
+        extra = np.array([lib.orc_table_size(int(c), a) - int(c) for c in cnt], np.int64)
+        roff = np.zeros(np_ + 1, np.int64)
+        roff[1:] = np.cumsum(extra)
+        assert np.array_equal(r.remap_offsets.astype(np.int64), roff)
