@@ -31,14 +31,16 @@ import phobic_hip as ph  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_bytes_per_key(name: str, lam: float, P: int) -> float:
+def stage_bytes_per_key(name: str, lam: float, P: int, key_bytes: float = 8.0) -> float:
     """Algorithmic bytes per key of each kernel: the rows of the reference's
-    stage structure that kernel performs (per-key figures: S1 8+1+4/l,
-    S2 12/l+1028/P, S3 8/l+8/P, S4 8+1+4/l+8, S5 8+1028/P+12/l+4/l)."""
-    s1 = 8 + 1 + 4 / lam
+    stage structure that kernel performs (per-key figures: S1 k+1+4/l,
+    S2 12/l+1028/P, S3 8/l+8/P, S4 k+1+4/l+8, S5 8+1028/P+12/l+4/l, where k
+    is the key's input bytes: 8 for u64 keys, the mean string length + its
+    8-byte offset for strings)."""
+    s1 = key_bytes + 1 + 4 / lam
     s2 = 12 / lam + 1028 / P
     s3_apply, s3_scan = 8 / lam, 8 / P
-    s4 = 8 + 1 + 4 / lam + 8
+    s4 = key_bytes + 1 + 4 / lam + 8
     s5 = 8 + 1028 / P + 16 / lam
     return {
         "hash_route": s1,               # hash + map; level-1 routing
@@ -264,6 +266,87 @@ def bench_serve(args) -> int:
     return 0 if out["failed"] == 0 else 1
 
 
+def bench_strings(args) -> int:
+    """Plain build of n distinct printable strings, lengths uniform in
+    [8, 64] (numpy seed 6), bytes generated in HBM, lambda 6, P 2048,
+    alpha 1."""
+    n, lam, P, alpha = args.strings, 6.0, 2048, 1.0
+    rng = np.random.default_rng(6)
+    offs = np.zeros(n + 1, np.uint64)
+    np.cumsum(rng.integers(8, 65, n, dtype=np.uint8), dtype=np.uint64, out=offs[1:])
+    total_bytes = int(offs[-1])
+    cfg = ph.make_config(lam, P, 0, alpha)
+    np_, B = ph.sizes(cfg, n)
+    ctx = ph.Context(0)
+    stream = hip_rt.Stream()
+    d_offs = hip_rt.DeviceBuffer.from_numpy(offs)
+    d_bytes = hip_rt.DeviceBuffer(total_bytes)
+    ph.gen_strings_device(6, n, d_offs, d_bytes, stream)
+    dp, do, ds = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
+    outs = ph.device_outputs(dp, do)
+    stream.synchronize()
+    for _ in range(args.warmup):
+        ctx.build_bytes_device(d_bytes, d_offs, n, cfg, outs, ds, stream)
+    stream.synchronize()
+    stage_sum: dict[str, float] = {}
+    e0, e1 = hip_rt.Event(), hip_rt.Event()
+    e0.record(stream)
+    for _ in range(args.steps):
+        ctx.build_bytes_device(d_bytes, d_offs, n, cfg, outs, ds, stream)
+        for k, v in ctx.last_stage_times().items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    e1.record(stream)
+    stream.synchronize()
+    status = int(ds.to_numpy(np.uint32)[0])
+    if status:
+        raise RuntimeError(f"build failed: {ph.STATUS_NAMES.get(ph.status_from_bits(status))}")
+    ms_per_step = e0.elapsed_ms(e1) / args.steps
+    stages = {k: v / args.steps for k, v in stage_sum.items()}
+    kernels = {k: v for k, v in stages.items() if k != "total"}
+    key_bytes = total_bytes / n + 8
+    out = {
+        "metric": "MPHF build Mkeys/s, 200M strings",
+        "value": round(n / (ms_per_step / 1e3) / 1e6, 2),
+        "unit": "Mkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": f"synthetic: {n} distinct printable strings, lengths U[8,64] (numpy seed 6), bytes generated in "
+                f"HBM ({total_bytes / 1e9:.2f} GB)",
+        "config": {"workload": f"plain PHOBIC build of {n} byte strings, lambda {lam}, partition size {P}, "
+                               f"alpha {alpha}, XXH3 seed 0, inputs and outputs resident in HBM",
+                   "n": n, "lambda": lam, "partition_size": P, "alpha": alpha, "partitions": np_,
+                   "buckets_per_partition": B, "mean_length": round(total_bytes / n, 2)},
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "stage_gbs": {k: round(stage_bytes_per_key(k, lam, P, key_bytes) * n / (v / 1e3) / 1e9, 1)
+                      for k, v in kernels.items()},
+        "tuning": ctx.tuning(),
+    }
+    if not args.no_cpu_baseline:
+        m = min(n, args.cpu_sample)
+        sample = d_bytes.to_numpy(np.uint8, int(offs[m]))
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+
+        t0 = time.perf_counter()
+        st = po.build_keys(po.hash_bytes(sample, offs[:m + 1], 0), lam, P)[0]
+        dt = time.perf_counter() - t0
+        if st != 0:
+            raise RuntimeError(f"oracle baseline failed with status {st}")
+        out["cpu_baseline"] = {
+            "value": round(m / dt / 1e6, 3), "unit": "Mkeys/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            "sample": f"the first {m} of the same strings: oracle XXH3 + build, OpenMP over partitions, {dt:.1f} s"}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
 def bench_copy(args) -> int:
     """A device-to-device copy of 1 GiB: the known-size reference for scaling
     rocprofv3 FETCH_SIZE / WRITE_SIZE readings to HBM bytes."""
@@ -299,7 +382,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
-    ap.add_argument("--mode", choices=["plain", "batch", "serve", "copy"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "strings", "batch", "serve", "copy"], default="plain")
+    ap.add_argument("--strings", type=int, default=200_000_000, help="string count of --mode strings")
     ap.add_argument("--sets", type=int, default=4096)
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--autotune", type=int, default=0,
@@ -320,6 +404,8 @@ def main() -> int:
         return bench_batch(args)
     if args.mode == "serve":
         return bench_serve(args)
+    if args.mode == "strings":
+        return bench_strings(args)
     if args.mode == "copy":
         return bench_copy(args)
     n, lam, P = args.n, args.lam, args.P

@@ -267,6 +267,11 @@ int phobic_check_bijection_device(phobic_ctx* ctx, const uint32_t* d_pos, uint64
  * (state += 0x9E3779B97F4A7C15, then the 30/27/31 mix).  Distinct by
  * construction. */
 int phobic_gen_splitmix64_device(uint64_t seed, uint64_t n, uint64_t* d_keys, void* stream);
+/* String i fills d_bytes[d_offsets[i], d_offsets[i+1]) (every length >= 8):
+ * its first 8 bytes are the hex digits of i's low 32 bits (so up to 2^32
+ * strings are distinct), the rest printable ASCII 0x20..0x7E from a
+ * splitmix64 stream seeded by (seed, i). */
+int phobic_gen_strings_device(uint64_t seed, uint64_t n, const uint64_t* d_offsets, uint8_t* d_bytes, void* stream);
 
 #ifdef __cplusplus
 }

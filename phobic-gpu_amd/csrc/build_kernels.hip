@@ -210,6 +210,38 @@ __global__ __launch_bounds__(256) void k_splitmix64(uint64_t seed, uint64_t n, u
   }
 }
 
+// Synthetic distinct printable strings: string i fills [offs[i], offs[i+1])
+// (lengths >= 8); its first 8 bytes are the hex digits of i's low 32 bits,
+// the rest printable ASCII 0x20..0x7E from a splitmix64 stream of (seed, i).
+__global__ __launch_bounds__(256) void k_gen_strings(uint64_t seed, uint64_t n, const uint64_t* __restrict__ offs,
+                                                    uint8_t* __restrict__ bytes) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const uint64_t a = offs[i], len = offs[i + 1] - a;
+    uint64_t state = seed ^ (i * 0xD1B54A32D192ED03ull);
+    uint64_t r = 0;
+    for (uint64_t j = 0; j < len; j++) {
+      uint8_t c;
+      if (j < 8) {
+        const uint32_t d = (uint32_t)(i >> (4 * j)) & 15u;
+        c = (uint8_t)(d < 10 ? '0' + d : 'a' + d - 10);
+      } else {
+        if ((j - 8) % 8 == 0) {
+          state += 0x9E3779B97F4A7C15ull;
+          uint64_t z = state;
+          z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+          z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+          r = z ^ (z >> 31);
+        }
+        c = (uint8_t)(0x20 + (uint32_t)((r & 0xFF) % 95u));
+        r >>= 8;
+      }
+      bytes[a + j] = c;
+    }
+  }
+}
+
 // ------------------------------------------------------------ launchers ---
 
 static inline uint32_t stream_grid(uint64_t n, uint32_t block, uint32_t max_blocks) {
@@ -265,6 +297,12 @@ hipError_t launch_check_bijection(const uint32_t* pos, uint64_t count, uint64_t 
 hipError_t launch_splitmix64(uint64_t seed, uint64_t n, uint64_t* out, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_splitmix64, dim3(stream_grid(n, 256, 256 * 32)), dim3(256), 0, s, seed, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_strings(uint64_t seed, uint64_t n, const uint64_t* offs, uint8_t* bytes, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gen_strings, dim3(stream_grid(n, 256, 256 * 8)), dim3(256), 0, s, seed, n, offs, bytes);
   return hipGetLastError();
 }
 

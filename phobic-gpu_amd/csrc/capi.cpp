@@ -991,4 +991,10 @@ int phobic_gen_splitmix64_device(uint64_t seed, uint64_t n, uint64_t* d_keys, vo
   return PHOBIC_OK;
 }
 
+int phobic_gen_strings_device(uint64_t seed, uint64_t n, const uint64_t* d_offsets, uint8_t* d_bytes, void* stream) {
+  if (n && (!d_offsets || !d_bytes)) return PHOBIC_E_INVALID_CONFIG;
+  TRY_HIP(launch_gen_strings(seed, n, d_offsets, d_bytes, (hipStream_t)stream));
+  return PHOBIC_OK;
+}
+
 }  // extern "C"

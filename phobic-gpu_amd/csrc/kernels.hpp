@@ -131,5 +131,6 @@ hipError_t launch_batch_offsets(const uint32_t* global_offsets, const uint32_t* 
 hipError_t launch_check_bijection(const uint32_t* pos, uint64_t count, uint64_t n, uint32_t* bitmap,
                                   unsigned long long* bad, hipStream_t s);
 hipError_t launch_splitmix64(uint64_t seed, uint64_t n, uint64_t* out, hipStream_t s);
+hipError_t launch_gen_strings(uint64_t seed, uint64_t n, const uint64_t* offs, uint8_t* bytes, hipStream_t s);
 
 }  // namespace phobic

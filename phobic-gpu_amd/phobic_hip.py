@@ -125,6 +125,7 @@ EXPORTS = [
     "phobic_eval_u64_device",
     "phobic_check_bijection_device",
     "phobic_gen_splitmix64_device",
+    "phobic_gen_strings_device",
 ]
 
 _lib = None
@@ -177,6 +178,7 @@ def load() -> ctypes.CDLL:
         "phobic_eval_u64_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
         "phobic_check_bijection_device": (I, [P, P, U64, U64, P, P, P]),
         "phobic_gen_splitmix64_device": (I, [U64, U64, P, P]),
+        "phobic_gen_strings_device": (I, [U64, U64, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -561,6 +563,15 @@ def gen_splitmix64_device(seed: int, n: int, d_keys, stream=None) -> None:
     st = lib.phobic_gen_splitmix64_device(seed, n, _ptr(d_keys), _stream(stream))
     if st:
         raise PhobicError(st, "gen_splitmix64_device")
+
+
+def gen_strings_device(seed: int, n: int, d_offsets, d_bytes, stream=None) -> None:
+    """Distinct printable strings in d_bytes at the ranges of d_offsets
+    (n + 1 u64, every length >= 8); see phobic_gen_strings_device."""
+    lib = load()
+    st = lib.phobic_gen_strings_device(seed, n, _ptr(d_offsets), _ptr(d_bytes), _stream(stream))
+    if st:
+        raise PhobicError(st, "gen_strings_device")
 
 
 def status_from_bits(bits: int) -> int:

@@ -300,3 +300,59 @@ def test_offsets_span_several_scan_tiles(ctx, alpha):
         roff = np.zeros(np_ + 1, np.int64)
         roff[1:] = np.cumsum(extra)
         assert np.array_equal(r.remap_offsets.astype(np.int64), roff)
+
+
+def gen_strings_ref(seed, offs):
+    """numpy restatement of phobic_gen_strings_device for small inputs."""
+    n = offs.size - 1
+    out = np.zeros(int(offs[-1]), np.uint8)
+    M = (1 << 64) - 1
+    for i in range(n):
+        a, ln = int(offs[i]), int(offs[i + 1] - offs[i])
+        state = seed ^ ((i * 0xD1B54A32D192ED03) & M)
+        r = 0
+        for j in range(ln):
+            if j < 8:
+                d = (i >> (4 * j)) & 15
+                c = 48 + d if d < 10 else 97 + d - 10
+            else:
+                if (j - 8) % 8 == 0:
+                    state = (state + 0x9E3779B97F4A7C15) & M
+                    z = state
+                    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+                    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+                    r = z ^ (z >> 31)
+                c = 0x20 + (r & 0xFF) % 95
+                r >>= 8
+            out[a + j] = c
+    return out
+
+
+def test_device_string_generator_and_build(ctx):
+    """Device-generated strings equal their numpy restatement, and a build on
+    them (lengths 8..64, lambda 6) equals the oracle's."""
+    rng = np.random.default_rng(12)
+    n = 2000
+    lens = rng.integers(8, 65, n)
+    offs = np.zeros(n + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    d_offs = hip_rt.DeviceBuffer.from_numpy(offs)
+    d_bytes = hip_rt.DeviceBuffer(int(offs[-1]))
+    ph.gen_strings_device(77, n, d_offs, d_bytes)
+    hip_rt.synchronize()
+    got = d_bytes.to_numpy(np.uint8, int(offs[-1]))
+    assert np.array_equal(got, gen_strings_ref(77, offs))
+    # a larger set for the build
+    n = 300_000
+    lens = rng.integers(8, 65, n)
+    offs = np.zeros(n + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    d_offs = hip_rt.DeviceBuffer.from_numpy(offs)
+    d_bytes = hip_rt.DeviceBuffer(int(offs[-1]))
+    ph.gen_strings_device(78, n, d_offs, d_bytes)
+    hip_rt.synchronize()
+    data = d_bytes.to_numpy(np.uint8, int(offs[-1]))
+    r = ctx.build_bytes(data, offs, ph.make_config(6.0, 2048))
+    st, opil, ooff = po.build_keys(po.hash_bytes(data, offs, 0), 6.0, 2048)
+    assert st == 0
+    assert np.array_equal(r.offsets, ooff) and np.array_equal(r.pilots, opil)
