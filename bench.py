@@ -50,21 +50,49 @@ def stage_bytes_per_key(name: str, lam: float, P: int, key_bytes: float = 8.0) -
     }.get(name, 0.0)
 
 
-def profiled_traffic(kernel: str, n: int, lam: float, P: int, alpha: float):
-    """HBM bytes per launch of `kernel` for this workload, as measured with
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known
-    size) and recorded in profiles/*.md; None when no profile covers it."""
+def profiled(kind: str, n: int, lam: float, P: int, alpha: float, **match) -> dict | None:
+    """Figures that rocprofv3 runs recorded in profiles/*.md for this
+    workload: lines `<kind> key=value ...` (HBM traffic from FETCH_SIZE /
+    WRITE_SIZE passes scaled by a copy kernel of known size, or the search
+    kernel's derived utilisation counters); None when no profile covers it."""
     import glob
 
-    want = {"kernel": kernel, "n": str(n), "lambda": str(lam), "P": str(P), "alpha": str(float(alpha))}
+    want = {"n": str(n), "lambda": str(lam), "P": str(P), "alpha": str(float(alpha))}
+    want.update({k: str(v) for k, v in match.items()})
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
-            if not line.startswith("traffic_bytes_per_launch "):
+            if not line.startswith(kind + " "):
                 continue
             kv = dict(tok.split("=", 1) for tok in line.split()[1:])
             if all(kv.get(k) == v for k, v in want.items()):
-                return float(kv["bytes"])
+                return kv
     return None
+
+
+def profiled_traffic(kernel: str, n: int, lam: float, P: int, alpha: float):
+    kv = profiled("traffic_bytes_per_launch", n, lam, P, alpha, kernel=kernel)
+    return float(kv["bytes"]) if kv else None
+
+
+def stage_report(kernels: dict, n: int, lam: float, P: int, alpha: float, key_bytes: float = 8.0) -> dict:
+    """Per stage: device ms, Mkeys/s, algorithmic bytes per key and GB/s with
+    their fraction of the 8 TB/s HBM peak, and the measured HBM GB/s where a
+    profile of this workload recorded the stage's traffic."""
+    rep = {}
+    for k, ms in kernels.items():
+        bpk = stage_bytes_per_key(k, lam, P, key_bytes)
+        gbs = bpk * n / (ms / 1e3) / 1e9
+        e = {"ms": round(ms, 4), "mkeys_s": round(n / (ms / 1e3) / 1e6, 1), "algo_bytes_per_key": round(bpk, 3),
+             "algo_gbs": round(gbs, 1), "algo_hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        tr = profiled("traffic_bytes_per_stage", n, lam, P, alpha, stage=k)
+        if tr is None and k == "partition_search":
+            tr = profiled("traffic_bytes_per_launch", n, lam, P, alpha, kernel=k)
+        if tr:
+            mg = float(tr["bytes"]) / (ms / 1e3) / 1e9
+            e["measured_gbs"] = round(mg, 1)
+            e["measured_hbm_frac"] = round(mg / HBM_PEAK_GBS, 4)
+        rep[k] = e
+    return rep
 
 
 def cpu_baseline(n_sample: int, lam: float, P: int, alpha: float, kseed: int) -> dict:
@@ -323,8 +351,7 @@ def bench_strings(args) -> int:
                    "n": n, "lambda": lam, "partition_size": P, "alpha": alpha, "partitions": np_,
                    "buckets_per_partition": B, "mean_length": round(total_bytes / n, 2)},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-        "stage_gbs": {k: round(stage_bytes_per_key(k, lam, P, key_bytes) * n / (v / 1e3) / 1e9, 1)
-                      for k, v in kernels.items()},
+        "stages": stage_report(kernels, n, lam, P, alpha, key_bytes),
         "tuning": ctx.tuning(),
     }
     if not args.no_cpu_baseline:
@@ -484,6 +511,7 @@ def main() -> int:
             "buckets_per_partition": B,
         },
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "stages": stage_report(kernels, n, lam, P, alpha),
         "tuning": ctx.tuning(),
         "autotuned": bool(tuning),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 3),
@@ -498,6 +526,10 @@ def main() -> int:
             "bytes_per_key": round(bpk, 3),
         },
     }
+    util = profiled("search_utilization", n, lam, P, alpha, kernel="partition_search")
+    if util:  # the search kernel is bound by its VALU issue, not by HBM
+        out["search_utilization"] = {k: float(v) for k, v in util.items()
+                                     if k in ("valu_busy", "salu_busy", "lds_util", "lds_bank_conflict")}
     if not args.no_verify:
         out_pos = hip_rt.DeviceBuffer(n * 4)
         ctx.eval_u64_device(dk, n, cfg, np_, mphf, out_pos, stream)
