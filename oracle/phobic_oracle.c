@@ -632,26 +632,131 @@ static uint64_t put_compact(uint8_t** o, const uint64_t* vals, uint64_t n) {
   return 32 + 8 * nw;
 }
 
-/* Serialised FastQueryMphf: header, fulcrums, B pilot columns
- * (interleaved_encoder.hpp:9-15: column j = pilots[j*np .. j*np+np)), then
- * the partition offsets as linear_diff_encoder (linear_diff_encoder.hpp:14-28:
- * v_i = offsets[i] - i*P, stored as (|v|<<1) | (v>0)). */
+/* Rice coding of a column (rice_sequence.hpp:14-46): parameter l from the
+ * mean (Kiely's rule, double precision as in optimalL, 0 for an all-zero
+ * column), low l bits in a compact vector of width l, v >> l in unary (zeros
+ * then a one). */
+uint64_t orc_rice_param(const uint64_t* v, uint64_t n) {
+  uint64_t sum = 0;
+  for (uint64_t i = 0; i < n; i++) sum += v[i];
+  if (sum == 0) return 0;
+  const double p = (double)n / ((double)sum + (double)n);
+  const double gold = log2(2.0 / (sqrt(5.0) + 1.0));
+  const double l = ceil(log2(gold / log2(1.0 - p)));
+  return l > 0 ? (uint64_t)l : 0;
+}
+
+/* A serialised Rice column: the unary bit vector (size, word count, words),
+ * its select index (darray.hpp:13-118: one count, blocks of 4096 ones as the
+ * first one's position or, when the block's ones span 2^16 bits or more,
+ * -1 - the index of its explicit positions; 16-bit offsets of every 64th one
+ * from the block start (0xFFFF in explicit blocks), packed four per word;
+ * the explicit positions) and the low bits (compact vector of width l).
+ * Returns the serialised byte count; *ref_bytes receives the reference's
+ * byte accounting (rice_sequence::bytes(), where 16-bit entries count 2). */
+static uint64_t put_rice(uint8_t** o, const uint64_t* v, uint64_t n, uint64_t* ref_bytes) {
+  const uint64_t l = orc_rice_param(v, n);
+  uint64_t* pos = (uint64_t*)malloc((n ? n : 1) * 8); /* position of value i's one */
+  uint64_t bits = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    bits += (l >= 64 ? 0 : v[i] >> l) + 1;
+    pos[i] = bits - 1;
+  }
+  const uint64_t hw = (bits + 63) / 64;
+  const uint64_t nb = (n + 4095) / 4096;
+  uint64_t ns = 0, nsp = 0;
+  for (uint64_t b = 0; b < nb; b++) {
+    const uint64_t lo = 4096 * b, hi = lo + 4096 < n ? lo + 4096 : n;
+    ns += (hi - lo + 63) / 64;
+    if (pos[hi - 1] - pos[lo] >= 65536) nsp += hi - lo;
+  }
+  const uint64_t lw = orc_compact_words(n, l);
+  if (*o) {
+    uint64_t* words = (uint64_t*)calloc(hw ? hw : 1, 8);
+    for (uint64_t i = 0; i < n; i++) words[pos[i] >> 6] |= 1ull << (pos[i] & 63);
+    *o = put64(*o, bits);
+    *o = put64(*o, hw);
+    for (uint64_t i = 0; i < hw; i++) *o = put64(*o, words[i]);
+    free(words);
+    *o = put64(*o, n);
+    *o = put64(*o, nb);
+    uint16_t* subs = (uint16_t*)calloc(ns ? ns : 1, 2);
+    uint64_t* sparse = (uint64_t*)malloc((nsp ? nsp : 1) * 8);
+    uint64_t si = 0, spi = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+      const uint64_t lo = 4096 * b, hi = lo + 4096 < n ? lo + 4096 : n;
+      const int dense = pos[hi - 1] - pos[lo] < 65536;
+      *o = put64(*o, dense ? pos[lo] : (uint64_t)(-(int64_t)spi - 1));
+      for (uint64_t i = lo; i < hi; i += 64) subs[si++] = dense ? (uint16_t)(pos[i] - pos[lo]) : 0xFFFF;
+      if (!dense)
+        for (uint64_t i = lo; i < hi; i++) sparse[spi++] = pos[i];
+    }
+    *o = put64(*o, ns);
+    for (uint64_t i = 0; i < ns; i += 4) {
+      uint64_t w = 0;
+      for (uint64_t k = 0; k < 4 && i + k < ns; k++) w |= (uint64_t)subs[i + k] << (16 * k);
+      *o = put64(*o, w);
+    }
+    *o = put64(*o, nsp);
+    for (uint64_t i = 0; i < nsp; i++) *o = put64(*o, sparse[i]);
+    free(subs);
+    free(sparse);
+    /* low bits: size, width, mask, word count, words */
+    uint64_t* low = (uint64_t*)calloc(lw, 8);
+    const uint64_t mask = l >= 64 ? ~0ull : ((1ull << l) - 1);
+    for (uint64_t i = 0; i < n && l; i++) {
+      const uint64_t x = v[i] & mask, p2 = i * l, blk = p2 >> 6, sh = p2 & 63;
+      low[blk] |= x << sh;
+      if (sh + l > 64) low[blk + 1] |= x >> (64 - sh);
+    }
+    *o = put64(*o, n);
+    *o = put64(*o, l);
+    *o = put64(*o, mask);
+    *o = put64(*o, lw);
+    for (uint64_t i = 0; i < lw; i++) *o = put64(*o, low[i]);
+    free(low);
+  }
+  free(pos);
+  *ref_bytes = (16 + 8 * hw) + (8 + (8 + 8 * nb) + (8 + 2 * ns) + (8 + 8 * nsp)) + (32 + 8 * lw);
+  return (16 + 8 * hw) + (8 + (8 + 8 * nb) + (8 + 8 * ((ns + 3) / 4)) + (8 + 8 * nsp)) + (32 + 8 * lw);
+}
+
+/* Serialised MPHF ("PHOBMPH2"): header (n, P, np, B, seed, pilot encoding,
+ * load factor ppm), fulcrums, B pilot columns (interleaved_encoder.hpp:9-15:
+ * column j = pilots[j*np .. j*np+np)) as compact vectors (encoding 0,
+ * FastQueryMphf) or Rice sequences (encoding 1, SmallSpaceMphf), then the
+ * partition offsets as linear_diff_encoder (linear_diff_encoder.hpp:14-28:
+ * v_i = offsets[i] - i*P, stored as (|v|<<1) | (v>0)).  *bits_per_key (if
+ * given) receives getBitsPerKey() (mphf.hpp:88-91) with the reference's byte
+ * accounting. */
 uint64_t orc_encode_mphf(const uint32_t* pilots, const uint32_t* offsets, uint32_t np, uint32_t B,
-                         uint32_t P, const uint32_t* fulcs, uint64_t n, uint8_t* out) {
+                         uint32_t P, const uint32_t* fulcs, uint64_t n, uint64_t seed, uint32_t encoding,
+                         uint8_t* out, double* bits_per_key) {
   uint8_t* o = out;
-  uint64_t total = 0;
-  o = put64(o, 0x3148504d424f4850ull); /* "PHOBMPH1" */
+  uint64_t total = 0, pilot_bytes = 0;
+  o = put64(o, 0x3248504d424f4850ull); /* "PHOBMPH2" */
   o = put64(o, n);
   o = put64(o, P);
   o = put64(o, np);
   o = put64(o, B);
-  total += 40;
+  o = put64(o, seed);
+  o = put64(o, encoding);
+  o = put64(o, 1000000u);
+  total += 64;
   for (int i = 0; i < ORC_FULCS; i += 2) o = put64(o, (uint64_t)fulcs[i] | ((uint64_t)fulcs[i + 1] << 32));
   total += ORC_FULCS * 4;
   uint64_t* col = (uint64_t*)malloc(((uint64_t)np + 1) * 8);
-  for (uint32_t j = 0; j < B; j++) {
+  for (uint32_t j = 0; j < B && np; j++) {
     for (uint32_t p = 0; p < np; p++) col[p] = pilots[(uint64_t)j * np + p];
-    total += np ? put_compact(&o, col, np) : 0;
+    if (encoding == 1) {
+      uint64_t rb = 0;
+      total += put_rice(&o, col, np, &rb);
+      pilot_bytes += rb;
+    } else {
+      const uint64_t cb = put_compact(&o, col, np);
+      total += cb;
+      pilot_bytes += cb;
+    }
   }
   for (uint32_t i = 0; i <= np; i++) {
     int64_t v = (int64_t)offsets[i] - (int64_t)i * (int64_t)P;
@@ -660,7 +765,10 @@ uint64_t orc_encode_mphf(const uint32_t* pilots, const uint32_t* offsets, uint32
   }
   o = put64(o, P);
   total += 8;
-  total += put_compact(&o, col, (uint64_t)np + 1);
+  const uint64_t ob = put_compact(&o, col, (uint64_t)np + 1);
+  total += ob;
   free(col);
+  if (bits_per_key)
+    *bits_per_key = n ? (double)(8 * pilot_bytes + 8 * ob + 32ull * ORC_FULCS + 32ull * np) / (double)n : 0.0;
   return total;
 }

@@ -98,11 +98,15 @@ uint64_t orc_compact_width(const uint64_t* values, uint64_t n);
 uint64_t orc_compact_encode(const uint64_t* values, uint64_t n, uint64_t* words, uint64_t* width_out);
 uint64_t orc_compact_access(const uint64_t* words, uint64_t width, uint64_t i);
 
-/* Serialised MPHF bytes (FastQueryMphf = interleaved_encoder<compact> +
- * diff_partition_encoder<compact>).  Returns the byte count; when out is
- * NULL only the size is computed. */
+/* Serialised MPHF bytes ("PHOBMPH2"; encoding 0 = FastQueryMphf's compact
+ * pilot columns, 1 = SmallSpaceMphf's Rice columns).  Returns the byte count;
+ * when out is NULL only the size is computed.  *bits_per_key, if non-NULL,
+ * receives the bits-per-key figure. */
 uint64_t orc_encode_mphf(const uint32_t* pilots, const uint32_t* offsets, uint32_t np, uint32_t B,
-                         uint32_t partition_size, const uint32_t* fulcs, uint64_t n, uint8_t* out);
+                         uint32_t P, const uint32_t* fulcs, uint64_t n, uint64_t seed, uint32_t encoding,
+                         uint8_t* out, double* bits_per_key);
+/* Rice parameter of a value sequence (rice_sequence.hpp:15-21). */
+uint64_t orc_rice_param(const uint64_t* v, uint64_t n);
 
 #ifdef __cplusplus
 }

@@ -47,7 +47,8 @@ def load() -> ctypes.CDLL:
         "orc_compact_width": (U64, [P, U64]),
         "orc_compact_encode": (U64, [P, U64, P, P]),
         "orc_compact_access": (U64, [P, U64, U64]),
-        "orc_encode_mphf": (U64, [P, P, U32, U32, U32, P, U64, P]),
+        "orc_encode_mphf": (U64, [P, P, U32, U32, U32, P, U64, U64, U32, P, P]),
+        "orc_rice_param": (U64, [P, U64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -189,7 +190,13 @@ def compact_access(words: np.ndarray, width: int, i: int) -> int:
     return load().orc_compact_access(np.ascontiguousarray(words).ctypes.data, width, i)
 
 
-def encode_mphf(pilots: np.ndarray, offsets: np.ndarray, lam: float, P: int, n: int) -> bytes:
+ENCODINGS = {"compact": 0, "rice": 1}
+
+
+def encode_mphf(pilots: np.ndarray, offsets: np.ndarray, lam: float, P: int, n: int, seed: int = 0,
+                encoding: str = "compact", with_bits_per_key: bool = False):
+    """Serialised MPHF bytes ("PHOBMPH2"); with_bits_per_key also returns the
+    bits-per-key figure."""
     lib = load()
     np_ = offsets.size - 1
     B = pilots.size // max(np_, 1) if np_ else sizes(n, lam, P)[1]
@@ -197,7 +204,15 @@ def encode_mphf(pilots: np.ndarray, offsets: np.ndarray, lam: float, P: int, n: 
     pil = np.ascontiguousarray(pilots, dtype=np.uint32)
     off = np.ascontiguousarray(offsets, dtype=np.uint32)
     pp = pil.ctypes.data if pil.size else np.zeros(1, np.uint32).ctypes.data
-    size = lib.orc_encode_mphf(pp, off.ctypes.data, np_, B, P, f.ctypes.data, n, None)
+    enc = ENCODINGS[encoding]
+    size = lib.orc_encode_mphf(pp, off.ctypes.data, np_, B, P, f.ctypes.data, n, seed, enc, None, None)
     out = np.zeros(size, dtype=np.uint8)
-    lib.orc_encode_mphf(pp, off.ctypes.data, np_, B, P, f.ctypes.data, n, out.ctypes.data)
-    return out.tobytes()
+    bpk = ctypes.c_double()
+    lib.orc_encode_mphf(pp, off.ctypes.data, np_, B, P, f.ctypes.data, n, seed, enc, out.ctypes.data,
+                        ctypes.byref(bpk))
+    return (out.tobytes(), bpk.value) if with_bits_per_key else out.tobytes()
+
+
+def rice_param(values: np.ndarray) -> int:
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    return int(load().orc_rice_param(v.ctypes.data if v.size else np.zeros(1, np.uint64).ctypes.data, v.size))

@@ -1,15 +1,18 @@
-// encode_driver.cpp -- runs the host half of the build (MPHF::setData with the
-// FastQueryMphf encoders, serialisation and queries) on pilots / offsets read
-// from a file.  Needs no GPU.
-//   encode_driver IN OUT [QUERY_KEYS QUERY_OUT]
+// encode_driver.cpp -- runs the host half of the build (MPHF::setData with
+// the FastQueryMphf or SmallSpaceMphf encoders, serialisation, loading and
+// queries) on pilots / offsets read from a file.  Needs no GPU.
+//   encode_driver [--small] IN OUT [QUERY_KEYS QUERY_OUT]
 // IN:  u64 n, u64 np, u64 B, u64 P, f64 lambda, u64 seed,
 //      u32 pilots[np*B] (interleaved), u32 offsets[np+1]
 // OUT: the serialised MPHF bytes.  QUERY_KEYS: u64 count + u64 keys;
-// QUERY_OUT: u32 positions.
+// QUERY_OUT: u32 positions, answered by the MPHF loaded back from OUT's bytes
+// (the driver fails if that copy serialises differently or answers
+// differently from the built one).
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include <phobic_gpu_mphf.hpp>
@@ -22,11 +25,8 @@ static void rd(std::ifstream& in, T* p, size_t count) {
   if (!in) throw std::runtime_error("short read");
 }
 
-int main(int argc, char** argv) {
-  if (argc != 3 && argc != 5) {
-    std::cerr << "usage: encode_driver IN OUT [QUERY_KEYS QUERY_OUT]" << std::endl;
-    return 2;
-  }
+template <typename Mphf>
+static int run(int argc, char** argv) {
   std::ifstream in(argv[1], std::ios::binary);
   uint64_t n, np, B, P, seed;
   double lambda;
@@ -45,11 +45,18 @@ int main(int argc, char** argv) {
     std::cerr << "bucket count mismatch" << std::endl;
     return 1;
   }
-  FastQueryMphf f;
+  Mphf f;
   f.setData(pilots, offsets, (uint32_t)np, cfg);
   const std::vector<uint8_t> bytes = f.serialize();
   std::ofstream out(argv[2], std::ios::binary);
   out.write(reinterpret_cast<const char*>(bytes.data()), bytes.size());
+  out.close();
+  Mphf g;  // the same MPHF loaded back
+  g.load(argv[2]);
+  if (g.serialize() != bytes) {
+    std::cerr << "loaded MPHF serialises differently" << std::endl;
+    return 1;
+  }
   if (argc == 5) {
     std::ifstream qin(argv[3], std::ios::binary);
     uint64_t count;
@@ -57,10 +64,29 @@ int main(int argc, char** argv) {
     std::vector<uint64_t> keys(count);
     rd(qin, keys.data(), count);
     std::vector<uint32_t> pos(count);
-    for (uint64_t i = 0; i < count; i++) pos[i] = f(keys[i]);
+    for (uint64_t i = 0; i < count; i++) {
+      pos[i] = g(keys[i]);
+      if (pos[i] != f(keys[i])) {
+        std::cerr << "loaded MPHF answers differently for key " << i << std::endl;
+        return 1;
+      }
+    }
     std::ofstream qout(argv[4], std::ios::binary);
     qout.write(reinterpret_cast<const char*>(pos.data()), pos.size() * 4);
   }
   std::printf("bits_per_key=%.4f bytes=%zu\n", n ? f.getBitsPerKey() : 0.0f, bytes.size());
   return 0;
+}
+
+int main(int argc, char** argv) {
+  bool small = argc > 1 && std::string(argv[1]) == "--small";
+  if (small) {
+    argv++;
+    argc--;
+  }
+  if (argc != 3 && argc != 5) {
+    std::cerr << "usage: encode_driver [--small] IN OUT [QUERY_KEYS QUERY_OUT]" << std::endl;
+    return 2;
+  }
+  return small ? run<SmallSpaceMphf>(argc, argv) : run<FastQueryMphf>(argc, argv);
 }

@@ -37,7 +37,22 @@ struct compact_vector {
     }
   }
 
+  // Fixed width w, 0 allowed (then every value reads as 0 and the vector is
+  // one guard word): the reference's compact_vector::builder(n, w) path.
+  template <typename Iterator>
+  void build_width(Iterator begin, uint64_t n, uint64_t w) {
+    if (w == 0) {
+      m_size = n;
+      m_width = 0;
+      m_mask = 0;
+      m_bits.assign(1, 0);
+      return;
+    }
+    build(begin, n, w);
+  }
+
   inline uint64_t access(uint64_t i) const {
+    if (!m_width) return 0;
     const uint64_t pos = i * m_width, blk = pos >> 6, sh = pos & 63;
     uint64_t v = m_bits[blk] >> sh;
     if (sh + m_width > 64) v |= m_bits[blk + 1] << (64 - sh);

@@ -38,7 +38,8 @@ struct linear_diff_encoder {
   template <typename Visitor>
   void visit(Visitor& visitor) {
     uint64_t inc = (uint64_t)increment;
-    visitor.visit(inc);
+    visitor.visit(inc);  // a loading visitor overwrites it
+    increment = (int64_t)inc;
     visitor.visit(enc);
   }
 

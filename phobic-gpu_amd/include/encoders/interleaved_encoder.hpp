@@ -10,6 +10,7 @@ namespace phobicgpu {
 
 template <typename BaseEncoder>
 struct interleaved_encoder {
+  static constexpr uint64_t encoding_id = BaseEncoder::encoding_id;
   template <typename Iterator>
   void encode(Iterator begin, uint64_t partitions, uint64_t buckets) {
     encoders.resize(buckets);
@@ -23,6 +24,8 @@ struct interleaved_encoder {
     for (auto& e : encoders) s += e.num_bits();
     return s;
   }
+  // one encoder per bucket column (before a loading visitor fills them)
+  void resize(uint64_t buckets) { encoders.resize(buckets); }
   template <typename Visitor>
   void visit(Visitor& visitor) {
     for (auto& e : encoders) e.visit(visitor);

@@ -5,6 +5,9 @@
 #pragma once
 #include <cstdint>
 #include <cstring>
+#include <fstream>
+#include <iterator>
+#include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -15,8 +18,9 @@
 
 namespace phobicgpu {
 
-// Writes the fields an encoder visits as little-endian u64 words; vectors
-// as their length followed by their words.
+// Writes the fields an encoder visits as little-endian u64 words; a vector
+// as its length followed by its elements (u64 / i64 one per word, u16 four
+// per word, the first in the low bits).
 struct ByteSaver {
   std::vector<uint8_t>& out;
   void put(uint64_t v) {
@@ -27,6 +31,57 @@ struct ByteSaver {
     put(v.size());
     for (uint64_t w : v) put(w);
   }
+  void visit(std::vector<int64_t>& v) {
+    put(v.size());
+    for (int64_t w : v) put(uint64_t(w));
+  }
+  void visit(std::vector<uint16_t>& v) {
+    put(v.size());
+    for (size_t i = 0; i < v.size(); i += 4) {
+      uint64_t w = 0;
+      for (size_t k = 0; k < 4 && i + k < v.size(); k++) w |= uint64_t(v[i + k]) << (16 * k);
+      put(w);
+    }
+  }
+  template <typename T>
+  void visit(T& t) {
+    t.visit(*this);
+  }
+};
+
+// Reads what ByteSaver wrote; throws std::runtime_error on truncated input.
+struct ByteLoader {
+  const uint8_t* p;
+  size_t left;
+  uint64_t get() {
+    if (left < 8) throw std::runtime_error("truncated MPHF bytes");
+    uint64_t v = 0;
+    for (int i = 0; i < 8; i++) v |= uint64_t(p[i]) << (8 * i);
+    p += 8;
+    left -= 8;
+    return v;
+  }
+  uint64_t length() {
+    const uint64_t n = get();
+    if (n > left) throw std::runtime_error("corrupt MPHF bytes");
+    return n;
+  }
+  void visit(uint64_t& v) { v = get(); }
+  void visit(std::vector<uint64_t>& v) {
+    v.resize(length());
+    for (auto& w : v) w = get();
+  }
+  void visit(std::vector<int64_t>& v) {
+    v.resize(length());
+    for (auto& w : v) w = int64_t(get());
+  }
+  void visit(std::vector<uint16_t>& v) {
+    v.resize(length());
+    for (size_t i = 0; i < v.size(); i += 4) {
+      const uint64_t w = get();
+      for (size_t k = 0; k < 4 && i + k < v.size(); k++) v[i + k] = uint16_t(w >> (16 * k));
+    }
+  }
   template <typename T>
   void visit(T& t) {
     t.visit(*this);
@@ -36,6 +91,8 @@ struct ByteSaver {
 template <typename PilotEncoder, typename PartitionOffsetEncoder, typename Hasher>
 class MPHF {
  private:
+  static constexpr uint64_t kMagic = 0x3248504d424f4850ull;  // "PHOBMPH2"
+
   std::vector<uint32_t> fulcs;
   uint32_t partitions = 0;
   uint32_t partitionSize = 0;
@@ -131,27 +188,82 @@ class MPHF {
 
   uint64_t size() const { return keys; }
 
-  // Encoded MPHF bytes: magic "PHOBMPH1", n, P, np, B, the 2048 fulcrums
-  // (u32 pairs per u64 word), the pilot encoder's fields, the partition
-  // offset encoder's fields (all little-endian u64).
+  // Encoded MPHF bytes (little-endian u64 words): magic "PHOBMPH2", n, P,
+  // np, B, the XXH3 seed, the pilot encoding (0 compact, 1 rice), the load
+  // factor in ppm (10^6 = alpha 1), the 2048 fulcrums as u32 pairs, the
+  // pilot columns, the partition offsets, and for alpha < 1 the remap
+  // offsets and entries.  The reference has no on-disk format.
   std::vector<uint8_t> serialize() {
     std::vector<uint8_t> out;
     ByteSaver s{out};
-    s.put(0x3148504d424f4850ull);
+    s.put(kMagic);
     s.put(keys);
     s.put(partitionSize);
     s.put(partitions);
     const uint64_t B = fulcs.empty() ? 0 : (fulcs.back() >> 16);
     s.put(B);
+    s.put(seed);
+    s.put(PilotEncoder::encoding_id);
+    s.put(hasRemap ? cfg.alphaPpm() : 1000000u);
     for (size_t i = 0; i + 1 < fulcs.size(); i += 2) s.put(uint64_t(fulcs[i]) | (uint64_t(fulcs[i + 1]) << 32));
     if (partitions) pilots.visit(s);
     partitionOffsets.visit(s);
-    if (hasRemap) {  // extension section, present only for alpha < 1
-      s.put(cfg.alphaPpm());
+    if (hasRemap) {
       remapOffsets.visit(s);
       remap.visit(s);
     }
     return out;
+  }
+
+  // The inverse of serialize(); throws std::runtime_error on bytes of
+  // another format, pilot encoding or length.
+  void deserialize(const uint8_t* data, size_t len) {
+    ByteLoader l{data, len};
+    if (l.get() != kMagic) throw std::runtime_error("not PHOBIC MPHF bytes");
+    keys = l.get();
+    partitionSize = (uint32_t)l.get();
+    partitions = (uint32_t)l.get();
+    const uint64_t B = l.get();
+    seed = l.get();
+    if (l.get() != PilotEncoder::encoding_id) throw std::runtime_error("MPHF bytes use another pilot encoding");
+    const uint64_t a_ppm = l.get();
+    fulcs.assign(FULCS_INTER, 0);
+    for (size_t i = 0; i < fulcs.size(); i += 2) {
+      const uint64_t w = l.get();
+      fulcs[i] = uint32_t(w);
+      fulcs[i + 1] = uint32_t(w >> 32);
+    }
+    cfg = MPHFconfig();
+    cfg.partitionSize = partitionSize;
+    cfg.bucketCountPerPartition = (uint32_t)B;
+    cfg.seed = seed;
+    cfg.alpha = double(a_ppm) / 1e6;
+    pilots = PilotEncoder();
+    if (partitions) {
+      pilots.resize(B);
+      pilots.visit(l);
+    }
+    partitionOffsets.visit(l);
+    hasRemap = a_ppm < 1000000u;
+    if (hasRemap) {
+      remapOffsets.visit(l);
+      remap.visit(l);
+    }
+    if (l.left) throw std::runtime_error("trailing MPHF bytes");
+  }
+
+  void save(const std::string& path) {
+    const std::vector<uint8_t> bytes = serialize();
+    std::ofstream out(path, std::ios::binary);
+    out.write(reinterpret_cast<const char*>(bytes.data()), (std::streamsize)bytes.size());
+    if (!out) throw std::runtime_error("cannot write " + path);
+  }
+
+  void load(const std::string& path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot read " + path);
+    const std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    deserialize(bytes.data(), bytes.size());
   }
 };
 

@@ -2,7 +2,9 @@
 // the same entry point as the reference's include/phobic_gpu_mphf.hpp:1-22.
 #pragma once
 
+#include "encoders/bit_vector.hpp"
 #include "encoders/compact_vector.hpp"
+#include "encoders/rice_sequence.hpp"
 #include "encoders/diff_partition_offset_encoder.hpp"
 #include "encoders/encoders.hpp"
 #include "encoders/interleaved_encoder.hpp"
@@ -16,5 +18,6 @@
 namespace phobicgpu {
 
 typedef MPHF<interleaved_encoder<compact>, diff_partition_encoder<compact>, xxhash> FastQueryMphf;
+typedef MPHF<interleaved_encoder<rice>, diff_partition_encoder<compact>, xxhash> SmallSpaceMphf;
 
 }  // namespace phobicgpu

@@ -311,38 +311,131 @@ def test_compact_vector_roundtrip(n, maxv):
 
 
 def _parse_mphf(buf):
+    """Decodes the serialised MPHF ("PHOBMPH2") without the oracle: header,
+    fulcrums, pilot columns (compact or Rice; Rice values are read by
+    scanning the unary bits, and the select index is rebuilt from the one
+    positions and compared), partition offsets."""
     v = np.frombuffer(buf, dtype="<u8")
-    magic, n, P, np_, B = (int(x) for x in v[:5])
-    at = 5
+    magic, n, P, np_, B, seed, enc, a_ppm = (int(x) for x in v[:8])
+    at = 8
     fl = v[at:at + 1024].view("<u4")
     at += 1024
 
     def compact(at):
         size, w, mask, nw = (int(x) for x in v[at:at + 4])
-        words = v[at + 4:at + 4 + nw]
-        return [po.compact_access(np.ascontiguousarray(words), w, i) for i in range(size)], at + 4 + nw
+        words = np.ascontiguousarray(v[at + 4:at + 4 + nw])
+        vals = [po.compact_access(words, w, i) for i in range(size)] if w else [0] * size
+        return vals, w, at + 4 + nw
 
-    cols = []
-    for _ in range(B):
-        c, at = compact(at)
+    def rice(at):
+        bits, hw = int(v[at]), int(v[at + 1])
+        words = [int(x) for x in v[at + 2:at + 2 + hw]]
+        at += 2 + hw
+        ones = [i for i in range(bits) if (words[i >> 6] >> (i & 63)) & 1]
+        count, nb = int(v[at]), int(v[at + 1])
+        blocks = [int(x) for x in v[at + 2:at + 2 + nb]]
+        at += 2 + nb
+        ns = int(v[at])
+        packed = v[at + 1:at + 1 + (ns + 3) // 4]
+        subs = [(int(packed[i // 4]) >> (16 * (i % 4))) & 0xFFFF for i in range(ns)]
+        at += 1 + (ns + 3) // 4
+        nsp = int(v[at])
+        sparse = [int(x) for x in v[at + 1:at + 1 + nsp]]
+        at += 1 + nsp
+        low, l, at = compact(at)
+        assert count == len(ones) == len(low)
+        eb, es, esp = [], [], []
+        for b0 in range(0, count, 4096):
+            blk = ones[b0:b0 + 4096]
+            if blk[-1] - blk[0] < 65536:
+                eb.append(blk[0])
+                es += [blk[i] - blk[0] for i in range(0, len(blk), 64)]
+            else:
+                eb.append((-len(esp) - 1) & (2**64 - 1))
+                esp += blk
+                es += [0xFFFF] * ((len(blk) + 63) // 64)
+        assert blocks == eb and subs == es and sparse == esp
+        highs = np.diff(np.array([-1] + ones)) - 1
+        sparse_entries.append(len(sparse))
+        return [(int(h) << l) | x for h, x in zip(highs, low)], l, at
+
+    cols, params, sparse_entries = [], [], []
+    for _ in range(B if np_ else 0):
+        c, w, at = compact(at) if enc == 0 else rice(at)
         cols.append(c)
+        params.append(w)
     inc = int(v[at])
-    diffs, at = compact(at + 1)
+    diffs, _, at = compact(at + 1)
     assert at == v.size
-    return magic, n, P, np_, B, fl, cols, inc, diffs
+    return dict(magic=magic, n=n, P=P, np=np_, B=B, seed=seed, enc=enc, a_ppm=a_ppm, fl=fl, cols=cols,
+                params=params, sparse=sparse_entries, inc=inc, diffs=diffs)
 
 
 def test_encoded_mphf_layout_decodes():
     lam, P = 7.0, 512
     hk = _keys(20000, seed=30)
     st, pil, off = po.build_keys(hk, lam, P)
-    buf = po.encode_mphf(pil, off, lam, P, hk.shape[0])
-    magic, n, P2, np_, B, fl, cols, inc, diffs = _parse_mphf(buf)
-    assert magic == 0x3148504D424F4850 and n == 20000 and P2 == P and inc == P
-    assert np.array_equal(fl, po.fulcrums(lam, P))
+    buf = po.encode_mphf(pil, off, lam, P, hk.shape[0], seed=7)
+    m = _parse_mphf(buf)
+    np_, B, cols, inc, diffs = m["np"], m["B"], m["cols"], m["inc"], m["diffs"]
+    assert m["magic"] == 0x3248504D424F4850 and m["n"] == 20000 and m["P"] == P and inc == P
+    assert m["seed"] == 7 and m["enc"] == 0 and m["a_ppm"] == 1000000
+    assert np.array_equal(m["fl"], po.fulcrums(lam, P))
     for j in range(B):
         assert cols[j] == [int(x) for x in pil[j * np_:(j + 1) * np_]]
     # linear_diff_encoder: value = (|v| << 1) | (v > 0), v = off[i] - i*P
     for i, e in enumerate(diffs):
         s = (2 * (e & 1) - 1) * (e >> 1)
         assert s + i * P == int(off[i])
+
+
+def test_rice_columns_decode_to_the_pilots():
+    lam, P = 7.0, 512
+    hk = _keys(20000, seed=31)
+    st, pil, off = po.build_keys(hk, lam, P)
+    buf, bpk = po.encode_mphf(pil, off, lam, P, hk.shape[0], encoding="rice", with_bits_per_key=True)
+    m = _parse_mphf(buf)
+    np_ = m["np"]
+    assert m["enc"] == 1
+    for j in range(m["B"]):
+        col = pil[j * np_:(j + 1) * np_].astype(np.uint64)
+        assert m["cols"][j] == [int(x) for x in col]
+        assert m["params"][j] == po.rice_param(col)
+    assert bpk > 0
+
+
+def test_rice_is_smaller_for_long_columns():
+    # ~1000 partitions per column: Rice codes beat fixed-width columns once
+    # the per-column index overhead is amortised
+    lam, P = 7.0, 512
+    hk = _keys(500_000, seed=32)
+    st, pil, off = po.build_keys(hk, lam, P)
+    _, rice_bpk = po.encode_mphf(pil, off, lam, P, hk.shape[0], encoding="rice", with_bits_per_key=True)
+    _, fast_bpk = po.encode_mphf(pil, off, lam, P, hk.shape[0], with_bits_per_key=True)
+    assert 0 < rice_bpk < fast_bpk
+
+
+def test_rice_sparse_select_blocks():
+    # two maximal values among zeros: their unary parts (2^32 >> l with
+    # l about 14) make their blocks of 4096 ones span more than 2^16 bits
+    n = 200_000
+    vals = np.zeros(n, np.uint64)
+    vals[[5000, 150000]] = 2**32 - 1
+    pil = vals.astype(np.uint32)  # one bucket column over n partitions
+    off = np.arange(n + 1, dtype=np.uint32)
+    buf = po.encode_mphf(pil, off, 1.0, 1, n, encoding="rice")
+    m = _parse_mphf(buf)
+    assert m["B"] == 1 and m["cols"][0] == [int(x) for x in vals]
+    assert m["sparse"][0] == 2 * 4096
+
+
+@pytest.mark.parametrize("mean", [0.3, 1.0, 4.0, 37.0, 500.0, 70000.0])
+def test_rice_parameter_near_optimal_for_geometric_values(mean):
+    """On geometric samples the chosen l gives a code length within 1% of
+    the best l found by trying every l (bits = n*(l+1) + sum(v >> l))."""
+    rng = np.random.default_rng(int(mean * 10))
+    vals = rng.geometric(1.0 / (mean + 1.0), 200_000).astype(np.uint64) - np.uint64(1)
+    cost = [len(vals) * (l + 1) + int((vals >> np.uint64(l)).sum()) for l in range(40)]
+    l = po.rice_param(vals)
+    assert cost[l] <= 1.01 * min(cost)
+    assert po.rice_param(np.zeros(100, np.uint64)) == 0
