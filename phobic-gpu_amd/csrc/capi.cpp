@@ -101,7 +101,7 @@ struct phobic_ctx {
   DevBuf in_a, in_b, out_pilots, out_offsets, out_remap, out_roff;
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
-  phobic_tuning tuning{2, 16, kMaxFanout, 0, 8, 32};
+  phobic_tuning tuning{2, 16, kMaxFanout, 0, 12, 32};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
@@ -883,7 +883,7 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   phobic_tuning win = saved;
   float part_ms = 1e30f, search_ms = 1e30f;
   for (uint32_t per : {8u, 12u, 16u})
-    for (uint32_t sper : {8u, 16u})
+    for (uint32_t sper : {8u, 12u, 16u})
       for (uint32_t f1 : {0u, 16u, 32u, 64u})
         for (uint32_t f : {64u, 128u}) {
           if (f1 > f) continue;

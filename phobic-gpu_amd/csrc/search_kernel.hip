@@ -433,21 +433,41 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   const uint32_t* Elane = E + lane;
   uint32_t err = 0;
 
-  uint64_t cur[4] = {0, 0, 0, 0};
+  const bool multi = !(a.mode & kSearchSinglePilot);
+  // Lane groups of a bucket of s <= 64 keys: 2^lg lanes per pilot (lg >= 2),
+  // 64 / 2^lg pilots per hash round; one group (lg = 6) for s > 32 or with
+  // single-pilot rounds.
+  auto lg_of = [&](uint32_t sz) -> uint32_t {
+    return multi && sz <= 32 ? (sz <= 4 ? 2u : 32u - __builtin_clz(sz - 1)) : 6u;
+  };
+  // A bucket's keys in registers: above 64 keys, key k in lane k & 63 of
+  // chunk k >> 6; otherwise lane l holds key min(l mod 2^lg, sz - 1), so
+  // every lane group holds the bucket (lanes past sz repeat its last key).
+  auto load_keys = [&](uint32_t st, uint32_t sz, uint64_t (&dst)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) dst[j] = 0;
+    if (sz == 0) return;
+    if (sz <= 64) {
+      const uint32_t k = lane & ((1u << lg_of(sz)) - 1);
+      dst[0] = sorted[st + min(k, sz - 1)];
+      return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t k = lane + 64 * j;
+      if (64 * j < sz && k < sz) dst[j] = sorted[st + k];
+    }
+  };
+  uint64_t cur[4];
   uint32_t cs = 0, cst = 0, cb = 0;
   if (m) {
     const uint2 e = reinterpret_cast<const uint2*>(rt)[0];
     cb = __builtin_amdgcn_readfirstlane(e.x) & 0xFFFFu;
     cs = __builtin_amdgcn_readfirstlane(e.x) >> 16;
     cst = __builtin_amdgcn_readfirstlane(e.y);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t k = lane + 64 * j;
-      if (64 * j < cs && k < cs) cur[j] = sorted[cst + k];
-    }
   }
+  load_keys(cst, cs, cur);
   uint32_t placed = 0, r = 0;
-  const bool multi = !(a.mode & kSearchSinglePilot);
 
   // Buckets r < r_end whose keys fit in C registers per lane (C chunks of
   // 64), scanned with H sub-windows per lane.  Buckets of at most 32 keys
@@ -471,27 +491,17 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         ns = __builtin_amdgcn_readfirstlane(e.x) >> 16;
         nst = __builtin_amdgcn_readfirstlane(e.y);
       }
-      // buckets after a C = 1 bucket have at most 64 keys too
-#pragma unroll
-      for (int j = 0; j < C; j++) {
-        const uint32_t k = lane + 64 * j;
-        if (64 * j < ns && k < ns) nxt[j] = sorted[nst + k];
-      }
+      load_keys(nst, ns, nxt);
 
       const uint32_t s = cs;
       // the d < 64 window test pays off for large buckets in a sparse table
       const bool win0 = s >= kWin0MinKeys && 2 * placed < tm;
-      uint32_t lg = 6;
-      if (C == 1 && multi && !win0 && s <= 32) lg = s <= 4 ? 2 : 32 - __builtin_clz(s - 1);
+      const uint32_t lg = C == 1 ? lg_of(s) : 6u;
       const uint32_t G = 64u >> lg;
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
-      uint64_t key[C];
-      if (C == 1) {
-        key[0] = __shfl(cur[0], (int)min(kk, s - 1), 64);
-      } else {
+      uint64_t key[C];  // already in the lane-group layout
 #pragma unroll
-        for (int j = 0; j < C; j++) key[j] = cur[j];
-      }
+      for (int j = 0; j < C; j++) key[j] = cur[j];
       uint32_t pk[C];  // packed slots
       uint32_t pilot = 0, dsel = 0;
       bool dupchk = false;
@@ -518,12 +528,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
           dlimit = (uint32_t)(0x80000000ull - vbase);
         }
         bool found = false;
-        if (win0) {  // G = 1: offsets 0..63, one window per key, OR over keys
+        if (win0) {  // offsets 0..63: one window per key of group gi, OR over keys
           uint64_t t0 = 0;
 #pragma unroll
           for (int j = 0; j < C; j++) {
-            const uint32_t k = lane + 64 * j;
-            if (64 * j < s && (kPad || k < s)) t0 |= window64(E, pk[j]);
+            const uint32_t k = kk + 64 * j;
+            if (g == gi && 64 * j < s && (kPad || k < s)) t0 |= window64(E, pk[j]);
           }
           const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
           const uint64_t free0 = ~taken0 & low_mask(dlimit);
@@ -591,7 +601,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       placed += s;
       lds_sync();
 #pragma unroll
-      for (int j = 0; j < C; j++) cur[j] = nxt[j];
+      for (int j = 0; j < 4; j++) cur[j] = nxt[j];
       cs = ns;
       cst = nst;
       cb = nb;
