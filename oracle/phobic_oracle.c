@@ -721,19 +721,37 @@ static uint64_t put_rice(uint8_t** o, const uint64_t* v, uint64_t n, uint64_t* r
   return (16 + 8 * hw) + (8 + (8 + 8 * nb) + (8 + 8 * ((ns + 3) / 4)) + (8 + 8 * nsp)) + (32 + 8 * lw);
 }
 
-/* Serialised MPHF ("PHOBMPH2"): header (n, P, np, B, seed, pilot encoding,
- * load factor ppm), fulcrums, B pilot columns (interleaved_encoder.hpp:9-15:
- * column j = pilots[j*np .. j*np+np)) as compact vectors (encoding 0,
- * FastQueryMphf) or Rice sequences (encoding 1, SmallSpaceMphf), then the
- * partition offsets as linear_diff_encoder (linear_diff_encoder.hpp:14-28:
- * v_i = offsets[i] - i*P, stored as (|v|<<1) | (v>0)).  *bits_per_key (if
+/* One base-encoded sequence (base 0 compact, 1 rice); returns the serialised
+ * bytes and adds the bytes num_bits() counts to *bpk_bytes. */
+static uint64_t put_seq(uint8_t** o, const uint64_t* v, uint64_t n, uint32_t base, uint64_t* bpk_bytes) {
+  if (base == 1) {
+    uint64_t rb = 0;
+    const uint64_t t = put_rice(o, v, n, &rb);
+    *bpk_bytes += rb;
+    return t;
+  }
+  const uint64_t cb = put_compact(o, v, n);
+  *bpk_bytes += cb;
+  return cb;
+}
+
+/* Serialised MPHF ("PHOBMPH2"): header (n, P, np, B, seed, encoding, load
+ * factor ppm), fulcrums, the pilots, then the partition offsets.  Pilots:
+ * B columns (interleaved_encoder.hpp:9-15: column j = pilots[j*np ..
+ * j*np+np)) as compact vectors (encoding 0, FastQueryMphf) or Rice sequences
+ * (encoding 1, SmallSpaceMphf); np then one sequence of all np*B pilots
+ * (encodings 2, 3); or the dual split point then columns of two bases
+ * (encodings 4..7).  Offsets: P then linear_diff_encoder
+ * (linear_diff_encoder.hpp:14-28: v_i = offsets[i] - i*P, stored as
+ * (|v|<<1) | (v>0)), or the offsets themselves (direct).  *bits_per_key (if
  * given) receives getBitsPerKey() (mphf.hpp:88-91) with the reference's byte
  * accounting. */
 uint64_t orc_encode_mphf(const uint32_t* pilots, const uint32_t* offsets, uint32_t np, uint32_t B,
                          uint32_t P, const uint32_t* fulcs, uint64_t n, uint64_t seed, uint32_t encoding,
                          uint8_t* out, double* bits_per_key) {
   uint8_t* o = out;
-  uint64_t total = 0, pilot_bytes = 0;
+  uint64_t total = 0, pilot_bytes = 0, off_bytes = 0;
+  const uint32_t pid = encoding & 0xFF, oid = (encoding >> 8) & 0xFF;
   o = put64(o, 0x3248504d424f4850ull); /* "PHOBMPH2" */
   o = put64(o, n);
   o = put64(o, P);
@@ -745,30 +763,44 @@ uint64_t orc_encode_mphf(const uint32_t* pilots, const uint32_t* offsets, uint32
   total += 64;
   for (int i = 0; i < ORC_FULCS; i += 2) o = put64(o, (uint64_t)fulcs[i] | ((uint64_t)fulcs[i + 1] << 32));
   total += ORC_FULCS * 4;
-  uint64_t* col = (uint64_t*)malloc(((uint64_t)np + 1) * 8);
-  for (uint32_t j = 0; j < B && np; j++) {
-    for (uint32_t p = 0; p < np; p++) col[p] = pilots[(uint64_t)j * np + p];
-    if (encoding == 1) {
-      uint64_t rb = 0;
-      total += put_rice(&o, col, np, &rb);
-      pilot_bytes += rb;
-    } else {
-      const uint64_t cb = put_compact(&o, col, np);
-      total += cb;
-      pilot_bytes += cb;
+  const uint64_t nb = (uint64_t)np * B;
+  uint64_t* col = (uint64_t*)malloc(((pid == 2 || pid == 3) ? nb + 1 : (uint64_t)np + 1) * 8);
+  if (np && (pid == 2 || pid == 3)) { /* one sequence in build order p + b*np (mono_encoders.hpp:14-18) */
+    o = put64(o, np);
+    total += 8;
+    for (uint64_t i = 0; i < nb; i++) col[i] = pilots[i];
+    total += put_seq(&o, col, nb, pid - 2, &pilot_bytes);
+  } else if (np) {
+    /* columns; dual: the first floor(B * 0.5f) with the first base
+     * (interleaved_encoder_dual.hpp:10-17), the split written first */
+    const int dual = pid >= 4;
+    const uint64_t b1 = dual ? (uint64_t)((float)B * 0.5f) : B;
+    if (dual) {
+      o = put64(o, b1);
+      total += 8;
+    }
+    for (uint32_t j = 0; j < B; j++) {
+      const uint32_t base = dual ? (j < b1 ? (pid - 4) >> 1 : (pid - 4) & 1) : pid;
+      for (uint32_t p = 0; p < np; p++) col[p] = pilots[(uint64_t)j * np + p];
+      total += put_seq(&o, col, np, base, &pilot_bytes);
     }
   }
-  for (uint32_t i = 0; i <= np; i++) {
-    int64_t v = (int64_t)offsets[i] - (int64_t)i * (int64_t)P;
-    uint64_t a = (uint64_t)(v < 0 ? -v : v);
-    col[i] = (a << 1) | (uint64_t)(v > 0);
+  if (oid >= 2) { /* offsets as they are (direct_partition_offset_encoder.hpp:14-17) */
+    for (uint32_t i = 0; i <= np; i++) col[i] = offsets[i];
+    total += put_seq(&o, col, (uint64_t)np + 1, oid - 2, &off_bytes);
+  } else {
+    for (uint32_t i = 0; i <= np; i++) {
+      int64_t v = (int64_t)offsets[i] - (int64_t)i * (int64_t)P;
+      uint64_t a = (uint64_t)(v < 0 ? -v : v);
+      col[i] = (a << 1) | (uint64_t)(v > 0);
+    }
+    o = put64(o, P);
+    total += 8;
+    total += put_seq(&o, col, (uint64_t)np + 1, oid, &off_bytes);
   }
-  o = put64(o, P);
-  total += 8;
-  const uint64_t ob = put_compact(&o, col, (uint64_t)np + 1);
-  total += ob;
   free(col);
   if (bits_per_key)
-    *bits_per_key = n ? (double)(8 * pilot_bytes + 8 * ob + 32ull * ORC_FULCS + 32ull * np) / (double)n : 0.0;
+    *bits_per_key =
+        n ? (double)(8 * pilot_bytes + 8 * off_bytes + 32ull * ORC_FULCS + 32ull * np) / (double)n : 0.0;
   return total;
 }

@@ -98,8 +98,12 @@ uint64_t orc_compact_width(const uint64_t* values, uint64_t n);
 uint64_t orc_compact_encode(const uint64_t* values, uint64_t n, uint64_t* words, uint64_t* width_out);
 uint64_t orc_compact_access(const uint64_t* words, uint64_t width, uint64_t i);
 
-/* Serialised MPHF bytes ("PHOBMPH2"; encoding 0 = FastQueryMphf's compact
- * pilot columns, 1 = SmallSpaceMphf's Rice columns).  Returns the byte count;
+/* Serialised MPHF bytes ("PHOBMPH2").  encoding: low byte the pilot layout
+ * (0 compact columns = FastQueryMphf, 1 Rice columns = SmallSpaceMphf, 2/3 one
+ * compact/Rice sequence, 4 + 2*first + second dual columns split at
+ * floor(B * 0.5f), so 6 = Rice then compact), next byte the offsets (0/1
+ * differences to i*P compact/Rice, 2/3 direct compact/Rice).  Returns the
+ * byte count;
  * when out is NULL only the size is computed.  *bits_per_key, if non-NULL,
  * receives the bits-per-key figure. */
 uint64_t orc_encode_mphf(const uint32_t* pilots, const uint32_t* offsets, uint32_t np, uint32_t B,

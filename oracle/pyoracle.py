@@ -190,11 +190,18 @@ def compact_access(words: np.ndarray, width: int, i: int) -> int:
     return load().orc_compact_access(np.ascontiguousarray(words).ctypes.data, width, i)
 
 
-ENCODINGS = {"compact": 0, "rice": 1}
+# pilot layouts: interleaved columns (FastQueryMphf compact, SmallSpaceMphf
+# rice), one sequence (mono_encoders.hpp), dual columns (the reference
+# benchmark's default interleaved_encoder_dual<rice, compact>, split 0.5)
+ENCODINGS = {"compact": 0, "rice": 1, "mono_compact": 2, "mono_rice": 3, "dual_rice_compact": 6}
+# partition offsets: differences to i*P (diff_partition_offset_encoder.hpp) or
+# the offsets themselves (direct_partition_offset_encoder.hpp)
+OFFSET_ENCODINGS = {"diff_compact": 0, "diff_rice": 1, "direct_compact": 2, "direct_rice": 3}
 
 
 def encode_mphf(pilots: np.ndarray, offsets: np.ndarray, lam: float, P: int, n: int, seed: int = 0,
-                encoding: str = "compact", with_bits_per_key: bool = False):
+                encoding: str = "compact", with_bits_per_key: bool = False,
+                offset_encoding: str = "diff_compact"):
     """Serialised MPHF bytes ("PHOBMPH2"); with_bits_per_key also returns the
     bits-per-key figure."""
     lib = load()
@@ -204,7 +211,7 @@ def encode_mphf(pilots: np.ndarray, offsets: np.ndarray, lam: float, P: int, n: 
     pil = np.ascontiguousarray(pilots, dtype=np.uint32)
     off = np.ascontiguousarray(offsets, dtype=np.uint32)
     pp = pil.ctypes.data if pil.size else np.zeros(1, np.uint32).ctypes.data
-    enc = ENCODINGS[encoding]
+    enc = ENCODINGS[encoding] | (OFFSET_ENCODINGS[offset_encoding] << 8)
     size = lib.orc_encode_mphf(pp, off.ctypes.data, np_, B, P, f.ctypes.data, n, seed, enc, None, None)
     out = np.zeros(size, dtype=np.uint8)
     bpk = ctypes.c_double()

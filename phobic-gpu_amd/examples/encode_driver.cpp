@@ -1,7 +1,11 @@
 // encode_driver.cpp -- runs the host half of the build (MPHF::setData with
-// the FastQueryMphf or SmallSpaceMphf encoders, serialisation, loading and
-// queries) on pilots / offsets read from a file.  Needs no GPU.
-//   encode_driver [--small] IN OUT [QUERY_KEYS QUERY_OUT]
+// a chosen pilot and offset encoder, serialisation, loading and queries) on
+// pilots / offsets read from a file.  Needs no GPU.
+//   encode_driver [--small] [--pilots NAME] [--offsets NAME] IN OUT [QUERY_KEYS QUERY_OUT]
+// --pilots: compact (FastQueryMphf, the default), rice (SmallSpaceMphf, also
+// --small), mono_compact, mono_rice, dual_rice_compact (the reference
+// benchmark's default, split 0.5); --offsets: diff_compact (the default),
+// diff_rice, direct_compact, direct_rice.
 // IN:  u64 n, u64 np, u64 B, u64 P, f64 lambda, u64 seed,
 //      u32 pilots[np*B] (interleaved), u32 offsets[np+1]
 // OUT: the serialised MPHF bytes.  QUERY_KEYS: u64 count + u64 keys;
@@ -78,15 +82,43 @@ static int run(int argc, char** argv) {
   return 0;
 }
 
+template <typename Pilots>
+static int with_offsets(const std::string& offsets, int argc, char** argv) {
+  if (offsets == "diff_compact") return run<MPHF<Pilots, diff_partition_encoder<compact>, xxhash>>(argc, argv);
+  if (offsets == "diff_rice") return run<MPHF<Pilots, diff_partition_encoder<rice>, xxhash>>(argc, argv);
+  if (offsets == "direct_compact") return run<MPHF<Pilots, direct_partition_encoder<compact>, xxhash>>(argc, argv);
+  if (offsets == "direct_rice") return run<MPHF<Pilots, direct_partition_encoder<rice>, xxhash>>(argc, argv);
+  std::cerr << "unknown offset encoder " << offsets << std::endl;
+  return 2;
+}
+
 int main(int argc, char** argv) {
-  bool small = argc > 1 && std::string(argv[1]) == "--small";
-  if (small) {
-    argv++;
-    argc--;
+  std::string pilots = "compact", offsets = "diff_compact";
+  while (argc > 1 && std::string(argv[1]).rfind("--", 0) == 0) {
+    const std::string flag = argv[1];
+    if (flag == "--small") {
+      pilots = "rice";
+      argv++;
+      argc--;
+    } else if ((flag == "--pilots" || flag == "--offsets") && argc > 2) {
+      (flag == "--pilots" ? pilots : offsets) = argv[2];
+      argv += 2;
+      argc -= 2;
+    } else {
+      std::cerr << "unknown flag " << flag << std::endl;
+      return 2;
+    }
   }
   if (argc != 3 && argc != 5) {
-    std::cerr << "usage: encode_driver [--small] IN OUT [QUERY_KEYS QUERY_OUT]" << std::endl;
+    std::cerr << "usage: encode_driver [--small] [--pilots NAME] [--offsets NAME] IN OUT [QUERY_KEYS QUERY_OUT]"
+              << std::endl;
     return 2;
   }
-  return small ? run<SmallSpaceMphf>(argc, argv) : run<FastQueryMphf>(argc, argv);
+  if (pilots == "compact") return with_offsets<interleaved_encoder<compact>>(offsets, argc, argv);
+  if (pilots == "rice") return with_offsets<interleaved_encoder<rice>>(offsets, argc, argv);
+  if (pilots == "mono_compact") return with_offsets<mono_encoder<compact>>(offsets, argc, argv);
+  if (pilots == "mono_rice") return with_offsets<mono_encoder<rice>>(offsets, argc, argv);
+  if (pilots == "dual_rice_compact") return with_offsets<interleaved_encoder_dual<rice, compact>>(offsets, argc, argv);
+  std::cerr << "unknown pilot encoder " << pilots << std::endl;
+  return 2;
 }

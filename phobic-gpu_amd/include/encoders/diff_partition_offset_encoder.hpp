@@ -50,6 +50,7 @@ struct linear_diff_encoder {
 
 template <typename BaseEncoder>
 struct diff_partition_encoder {
+  static constexpr uint64_t encoding_id = BaseEncoder::encoding_id;  // in the serialised MPHF header
   template <typename Iterator>
   void encode(Iterator begin, uint64_t partitionSize, uint64_t partitions) {
     enc.encode(begin, partitions, partitionSize);

@@ -188,11 +188,18 @@ class MPHF {
 
   uint64_t size() const { return keys; }
 
+  // Encoding field of the serialised header: the pilot encoder's id in the
+  // low byte (interleaved columns: 0 compact, 1 rice; one sequence: 2
+  // compact, 3 rice; dual columns: 4 + 2*first + second, so 6 for
+  // rice-then-compact), the offset encoder's id in the next byte (differences
+  // to i*P: 0 compact, 1 rice; direct: 2 compact, 3 rice).
+  static constexpr uint64_t layoutId() { return PilotEncoder::encoding_id | (PartitionOffsetEncoder::encoding_id << 8); }
+
   // Encoded MPHF bytes (little-endian u64 words): magic "PHOBMPH2", n, P,
-  // np, B, the XXH3 seed, the pilot encoding (0 compact, 1 rice), the load
-  // factor in ppm (10^6 = alpha 1), the 2048 fulcrums as u32 pairs, the
-  // pilot columns, the partition offsets, and for alpha < 1 the remap
-  // offsets and entries.  The reference has no on-disk format.
+  // np, B, the XXH3 seed, the encoding (layoutId()), the load factor in ppm
+  // (10^6 = alpha 1), the 2048 fulcrums as u32 pairs, the pilots, the
+  // partition offsets, and for alpha < 1 the remap offsets and entries.  The
+  // reference has no on-disk format.
   std::vector<uint8_t> serialize() {
     std::vector<uint8_t> out;
     ByteSaver s{out};
@@ -203,7 +210,7 @@ class MPHF {
     const uint64_t B = fulcs.empty() ? 0 : (fulcs.back() >> 16);
     s.put(B);
     s.put(seed);
-    s.put(PilotEncoder::encoding_id);
+    s.put(layoutId());
     s.put(hasRemap ? cfg.alphaPpm() : 1000000u);
     for (size_t i = 0; i + 1 < fulcs.size(); i += 2) s.put(uint64_t(fulcs[i]) | (uint64_t(fulcs[i + 1]) << 32));
     if (partitions) pilots.visit(s);
@@ -225,7 +232,7 @@ class MPHF {
     partitions = (uint32_t)l.get();
     const uint64_t B = l.get();
     seed = l.get();
-    if (l.get() != PilotEncoder::encoding_id) throw std::runtime_error("MPHF bytes use another pilot encoding");
+    if (l.get() != layoutId()) throw std::runtime_error("MPHF bytes use another pilot or offset encoding");
     const uint64_t a_ppm = l.get();
     fulcs.assign(FULCS_INTER, 0);
     for (size_t i = 0; i < fulcs.size(); i += 2) {

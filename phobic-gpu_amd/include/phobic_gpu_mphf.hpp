@@ -6,8 +6,11 @@
 #include "encoders/compact_vector.hpp"
 #include "encoders/rice_sequence.hpp"
 #include "encoders/diff_partition_offset_encoder.hpp"
+#include "encoders/direct_partition_offset_encoder.hpp"
 #include "encoders/encoders.hpp"
 #include "encoders/interleaved_encoder.hpp"
+#include "encoders/interleaved_encoder_dual.hpp"
+#include "encoders/mono_encoder.hpp"
 #include "phobicGpu/bucketers.h"
 #include "phobicGpu/hasher.hpp"
 #include "phobicGpu/host_timer.h"
