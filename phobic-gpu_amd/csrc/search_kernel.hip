@@ -134,13 +134,14 @@ __device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t M, uint32_t d) 
 // a mod d for a < 2^31 and 512 <= d < 2^24 without 32-bit integer
 // multiplies (quarter rate): the f32 quotient estimate q has a relative
 // error below 3 * 2^-24, so for a / d < 2^22 it is off by at most one, and
-// a - q*d (exact as a 24x24-bit product) needs one correction either way.
+// r = a - q*d (mod 2^32, a 24x24-bit product) is the remainder plus d, the
+// remainder, or the remainder minus d (wrapped).  Of r, r + d and r - d the
+// smallest as unsigned is then the remainder: the others are at least d or
+// wrapped past 2^32 - 2d.
 __device__ __forceinline__ uint32_t mod_f32(uint32_t a, float rd, uint32_t d) {
   const uint32_t q = (uint32_t)(__uint2float_rn(a) * rd);
-  int32_t r = (int32_t)(a - __umul24(q, d));
-  r += r < 0 ? (int32_t)d : 0;
-  r -= r >= (int32_t)d ? (int32_t)d : 0;
-  return (uint32_t)r;
+  const uint32_t r = a - __umul24(q, d);
+  return min(min(r, r + d), r - d);
 }
 
 __device__ __forceinline__ uint64_t low_mask(uint32_t bits) {  // bits >= 64 -> all ones
@@ -150,7 +151,7 @@ __device__ __forceinline__ uint64_t low_mask(uint32_t bits) {  // bits >= 64 -> 
 
 // Packed slot q: (q >> 5) << 7 | (q & 31).  Shifted right by 5 it is the
 // byte offset of q's bitmap word; v_alignbit reads only its low 5 bits.
-__device__ __forceinline__ uint32_t pack_slot(uint32_t q) { return ((q & ~31u) << 2) | (q & 31); }
+__device__ __forceinline__ uint32_t pack_slot(uint32_t q) { return q + __umul24(q & ~31u, 3u); }
 __device__ __forceinline__ uint32_t unpack_slot(uint32_t x) { return ((x >> 2) & ~31u) | (x & 31); }
 
 // Mask of the offsets [off, off + 32) below lim.
@@ -505,19 +506,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       uint32_t pk[C];  // packed slots
       uint32_t pilot = 0, dsel = 0;
       bool dupchk = false;
+      auto slot = [&](uint64_t kv, uint32_t pl) {
+        const uint32_t h = mix32((uint32_t)kv, mix32((uint32_t)(kv >> 32), pl)) >> 1;
+        return pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
+      };
       for (;; pilot++) {
-        const uint32_t gi = pilot & (G - 1);
-        if (gi == 0) {  // slots for pilots pilot .. pilot + G - 1
-#pragma unroll
-          for (int j = 0; j < C; j++) {
-            const uint32_t k = kk + 64 * j;
-            pk[j] = 0;
-            if (64 * j < s && (kPad || k < s)) {
-              const uint32_t h = mix32((uint32_t)key[j], mix32((uint32_t)(key[j] >> 32), pilot + g)) >> 1;
-              pk[j] = pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
-            }
-          }
-        }
         uint32_t dlimit = tm;  // d < dlimit keeps V = d + tm * pilot below 2^31
         if (pilot > pilot_full) {
           const uint64_t vbase = (uint64_t)tm * pilot;
@@ -527,58 +520,86 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
           }
           dlimit = (uint32_t)(0x80000000ull - vbase);
         }
-        bool found = false;
-        if (win0) {  // offsets 0..63: one window per key of group gi, OR over keys
-          uint64_t t0 = 0;
+        const uint32_t gi = pilot & (G - 1);
+        bool coincide = false;  // two slots of the bucket coincide (wave-uniform)
+        if constexpr (C > 1) {
+          // Above 64 keys most pilots fail because two slots coincide: the
+          // chunks of 64 keys enter the collision bitmap one at a time and
+          // the pilot is dropped at the first coincidence, before its
+          // offsets are scanned.
+          uint32_t jn = 0;
 #pragma unroll
           for (int j = 0; j < C; j++) {
-            const uint32_t k = kk + 64 * j;
-            if (g == gi && 64 * j < s && (kPad || k < s)) t0 |= window64(E, pk[j]);
+            if (!coincide && 64 * j < s) {
+              const uint32_t k = lane + 64 * j;
+              bool cj = false;
+              pk[j] = 0;
+              if (k < s) {
+                pk[j] = slot(key[j], pilot);
+                const uint32_t msk = 1u << (pk[j] & 31);
+                cj = (atomicOr(&coll[pk[j] >> 7], msk) & msk) != 0;
+              }
+              lds_sync();
+              coincide = __any(cj);
+              jn = j + 1;
+            }
           }
-          const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
-          const uint64_t free0 = ~taken0 & low_mask(dlimit);
-          if (free0) {
-            dsel = (uint32_t)__builtin_ctzll(free0);
-            found = true;
+#pragma unroll
+          for (int j = 0; j < C; j++)
+            if ((uint32_t)j < jn && lane + 64 * j < s) coll[pk[j] >> 7] = 0;
+          lds_sync();
+        } else if (gi == 0) {  // slots for pilots pilot .. pilot + G - 1
+          pk[0] = slot(key[0], pilot + g);
+        }
+        if (!coincide) {
+          bool found = false;
+          if (win0) {  // offsets 0..63: one window per key of group gi, OR over keys
+            uint64_t t0 = 0;
+#pragma unroll
+            for (int j = 0; j < C; j++) {
+              const uint32_t k = kk + 64 * j;
+              if (g == gi && 64 * j < s && (kPad || k < s)) t0 |= window64(E, pk[j]);
+            }
+            const uint64_t taken0 = ((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0);
+            const uint64_t free0 = ~taken0 & low_mask(dlimit);
+            if (free0) {
+              dsel = (uint32_t)__builtin_ctzll(free0);
+              found = true;
+            }
+          }
+          if (!found && !scan_offsets<C, H, kPad>(Elane, pk, gi << lg, s, dlimit, dlimit == tm, vfull, dsel)) continue;
+          if constexpr (C == 1) {
+            if (s > 1) {  // the slots must be pairwise distinct
+              bool c = false;
+              if (g == gi && kk < s) {
+                const uint32_t msk = 1u << (pk[0] & 31);
+                c = (atomicOr(&coll[pk[0] >> 7], msk) & msk) != 0;
+              }
+              lds_sync();
+              if (g == gi && kk < s) coll[pk[0] >> 7] = 0;
+              lds_sync();
+              coincide = __any(c);
+            }
           }
         }
-        if (!found && !scan_offsets<C, H, kPad>(Elane, pk, gi << lg, s, dlimit, dlimit == tm, vfull, dsel)) continue;
-        if (s > 1) {  // the slots must be pairwise distinct
-          bool c = false;
+        if (coincide) {
+          if (pilot >= kDupCheckPilot && !dupchk) {
+            dupchk = true;
+            bool dup = false;
+            for (uint32_t k2 = 0; k2 < s; k2++) {
+              const uint64_t kv = sorted[cst + k2];
 #pragma unroll
-          for (int j = 0; j < C; j++) {
-            const uint32_t k = kk + 64 * j;
-            if (g == gi && 64 * j < s && k < s) {
-              const uint32_t msk = 1u << (pk[j] & 31);
-              c |= (atomicOr(&coll[pk[j] >> 7], msk) & msk) != 0;
-            }
-          }
-          lds_sync();
-#pragma unroll
-          for (int j = 0; j < C; j++) {
-            const uint32_t k = kk + 64 * j;
-            if (g == gi && 64 * j < s && k < s) coll[pk[j] >> 7] = 0;
-          }
-          lds_sync();
-          if (__any(c)) {
-            if (pilot >= kDupCheckPilot && !dupchk) {
-              dupchk = true;
-              bool dup = false;
-              for (uint32_t k2 = 0; k2 < s; k2++) {
-                const uint64_t kv = sorted[cst + k2];
-#pragma unroll
-                for (int j = 0; j < C; j++) {
-                  const uint32_t k = lane + 64 * j;
-                  if (k < s && k != k2 && cur[j] == kv) dup = true;
-                }
-              }
-              if (__any(dup)) {
-                err = kErrDuplicateKeys;
-                break;
+              for (int j = 0; j < C; j++) {
+                const uint32_t k = lane + 64 * j;
+                if (k < s && k != k2 && cur[j] == kv) dup = true;
               }
             }
-            continue;
+            if (__any(dup)) {
+              err = kErrDuplicateKeys;
+              break;
+            }
           }
+          continue;
         }
         break;
       }
