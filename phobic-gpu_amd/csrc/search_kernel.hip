@@ -159,34 +159,58 @@ __device__ __forceinline__ uint32_t below_mask(uint32_t off, uint32_t lim) {
   return off >= lim ? 0u : lim - off >= 32 ? ~0u : (1u << (lim - off)) - 1;
 }
 
-// Smallest offset d < dlimit at which the s slots held by lanes
-// [gbase, gbase + s) of pk (packed, C chunks of 64 lanes) are all free, or
-// false.  One pass covers 2048 * H offsets: lane l ORs the 32-bit windows
-// of every key at o0 + 2048 h + 32 l (consecutive words across the lanes,
-// read kStep keys at a time so that the LDS reads overlap).  With kPad the
-// lanes [gbase + s, gbase + round_up(s, 4)) repeat a slot of the bucket, so
-// the key loop needs no clamp; `vfull` is the first pass's offset mask for
-// dlimit = tm.
-template <int C, int H, bool kPad>
-__device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32_t (&pk)[C], uint32_t gbase,
-                                             uint32_t s, uint32_t dlimit, bool full, const uint32_t (&vfull)[H],
-                                             uint32_t& dsel) {
-  constexpr uint32_t kStep = (C > 1 || H >= 4) ? 2 : 4;
-  const uint32_t lane = __lane_id();
-  if (full && dlimit <= 2048 * H) {  // one pass, offset masks known
-    uint32_t acc[H];
+// ORs into acc[h] the 32-bit windows at o0 + 2048 h + 32 lane (eb = the
+// lane's E word of offset o0) of the N slots held by lanes lane0 ..
+// lane0 + N - 1 of pk (packed); the N reads are issued before the first use.
+template <int H, uint32_t N>
+__device__ __forceinline__ void or_windows(const char* eb, uint32_t pk, uint32_t lane0, uint32_t (&acc)[H]) {
+  uint32_t x[N], lo[N][H], hi[N][H];
 #pragma unroll
-    for (int h = 0; h < H; h++) acc[h] = 0;
+  for (uint32_t i = 0; i < N; i++) {
+    x[i] = __builtin_amdgcn_readlane(pk, lane0 + i);
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(eb + (x[i] >> 5));
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+      lo[i][h] = ew[64 * h];
+      hi[i][h] = ew[64 * h + 1];
+    }
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < N; i++)
+#pragma unroll
+    for (int h = 0; h < H; h++) acc[h] |= __builtin_amdgcn_alignbit(hi[i][h], lo[i][h], x[i]);
+}
+
+// The windows of all s slots of a bucket: lanes [gbase, gbase + s) of pk[0]
+// when C = 1 (4 or 2, then 2, then 1 at a time, so no slot is read twice), or
+// lane k & 63 of pk[k >> 6] when C > 1 (2 at a time, the last slot repeated
+// when one is left; OR is idempotent).
+template <int C, int H>
+__device__ __forceinline__ void or_bucket(const char* eb, const uint32_t (&pk)[C], uint32_t gbase, uint32_t s,
+                                          uint32_t (&acc)[H]) {
+  if constexpr (C == 1 && H < 4) {
+    uint32_t k = 0;
+    for (; k + 4 <= s; k += 4) or_windows<H, 4>(eb, pk[0], gbase + k, acc);
+    if (s & 2) {
+      or_windows<H, 2>(eb, pk[0], gbase + k, acc);
+      k += 2;
+    }
+    if (s & 1) or_windows<H, 1>(eb, pk[0], gbase + k, acc);
+  } else if constexpr (C == 1) {  // four sub-windows: 2 slots at a time keep the registers low
+    uint32_t k = 0;
+    for (; k + 2 <= s; k += 2) or_windows<H, 2>(eb, pk[0], gbase + k, acc);
+    if (s & 1) or_windows<H, 1>(eb, pk[0], gbase + k, acc);
+  } else {
 #pragma unroll
     for (int j = 0; j < C; j++) {
       if (64 * j < s) {
         const uint32_t kl = min(s - 64 * j, 64u) - 1;
-        for (uint32_t k = 0; k <= kl; k += kStep) {
-          uint32_t x[kStep], lo[kStep][H], hi[kStep][H];
+        for (uint32_t k = 0; k <= kl; k += 2) {
+          uint32_t x[2], lo[2][H], hi[2][H];
 #pragma unroll
-          for (uint32_t i = 0; i < kStep; i++) {
-            x[i] = __builtin_amdgcn_readlane(pk[j], kPad ? gbase + k + i : gbase + min(k + i, kl));
-            const uint32_t* ew = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(Elane) + (x[i] >> 5));
+          for (uint32_t i = 0; i < 2; i++) {
+            x[i] = __builtin_amdgcn_readlane(pk[j], gbase + min(k + i, kl));
+            const uint32_t* ew = reinterpret_cast<const uint32_t*>(eb + (x[i] >> 5));
 #pragma unroll
             for (int h = 0; h < H; h++) {
               lo[i][h] = ew[64 * h];
@@ -194,12 +218,31 @@ __device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32
             }
           }
 #pragma unroll
-          for (uint32_t i = 0; i < kStep; i++)
+          for (uint32_t i = 0; i < 2; i++)
 #pragma unroll
             for (int h = 0; h < H; h++) acc[h] |= __builtin_amdgcn_alignbit(hi[i][h], lo[i][h], x[i]);
         }
       }
     }
+  }
+}
+
+// Smallest offset d < dlimit at which the s slots of the bucket (held in pk
+// as or_bucket reads them) are all free, or false.  One pass covers
+// 2048 * H offsets: lane l ORs the 32-bit windows of every slot at
+// o0 + 2048 h + 32 l (consecutive words across the lanes); the lowest free
+// bit of the lowest lane is the smallest offset.  `vfull` is the first
+// pass's offset mask for dlimit = tm.
+template <int C, int H>
+__device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32_t (&pk)[C], uint32_t gbase,
+                                             uint32_t s, uint32_t dlimit, bool full, const uint32_t (&vfull)[H],
+                                             uint32_t& dsel) {
+  const uint32_t lane = __lane_id();
+  if (full && dlimit <= 2048 * H) {  // one pass, offset masks known
+    uint32_t acc[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) acc[h] = 0;
+    or_bucket<C, H>(reinterpret_cast<const char*>(Elane), pk, gbase, s, acc);
 #pragma unroll
     for (int h = 0; h < H; h++) {
       const uint32_t fr = ~acc[h] & vfull[h];
@@ -213,35 +256,10 @@ __device__ __forceinline__ bool scan_offsets(const uint32_t* Elane, const uint32
     return false;
   }
   for (uint32_t o0 = 0; o0 < dlimit; o0 += 2048 * H) {
-    const char* eb = reinterpret_cast<const char*>(Elane + (o0 >> 5));
     uint32_t acc[H];
 #pragma unroll
     for (int h = 0; h < H; h++) acc[h] = 0;
-#pragma unroll
-    for (int j = 0; j < C; j++) {
-      if (64 * j < s) {
-        // without padding the last key repeats when fewer than kStep are
-        // left (OR is idempotent)
-        const uint32_t kl = min(s - 64 * j, 64u) - 1;
-        for (uint32_t k = 0; k <= kl; k += kStep) {
-          uint32_t x[kStep], lo[kStep][H], hi[kStep][H];
-#pragma unroll
-          for (uint32_t i = 0; i < kStep; i++) {
-            x[i] = __builtin_amdgcn_readlane(pk[j], kPad ? gbase + k + i : gbase + min(k + i, kl));
-            const uint32_t* ew = reinterpret_cast<const uint32_t*>(eb + (x[i] >> 5));
-#pragma unroll
-            for (int h = 0; h < H; h++) {
-              lo[i][h] = ew[64 * h];
-              hi[i][h] = ew[64 * h + 1];
-            }
-          }
-#pragma unroll
-          for (uint32_t i = 0; i < kStep; i++)
-#pragma unroll
-            for (int h = 0; h < H; h++) acc[h] |= __builtin_amdgcn_alignbit(hi[i][h], lo[i][h], x[i]);
-        }
-      }
-    }
+    or_bucket<C, H>(reinterpret_cast<const char*>(Elane + (o0 >> 5)), pk, gbase, s, acc);
 #pragma unroll
     for (int h = 0; h < H; h++) {
       const uint32_t valid = (full && o0 == 0) ? vfull[h] : below_mask(o0 + 2048 * h + 32 * lane, dlimit);
@@ -567,7 +585,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
               found = true;
             }
           }
-          if (!found && !scan_offsets<C, H, kPad>(Elane, pk, gi << lg, s, dlimit, dlimit == tm, vfull, dsel)) continue;
+          if (!found && !scan_offsets<C, H>(Elane, pk, gi << lg, s, dlimit, dlimit == tm, vfull, dsel)) continue;
           if constexpr (C == 1) {
             if (s > 1) {  // the slots must be pairwise distinct
               bool c = false;
