@@ -148,11 +148,6 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
   };
   if (KIND == kKindU64 && blockIdx.x < ntiles) load_u64(blockIdx.x, kv);
   for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    uint64_t nk[kPer];
-    if (KIND == kKindU64 && t + gridDim.x < ntiles) load_u64(t + gridDim.x, nk);
-    uint64_t lo[kPer];
-    uint32_t meta[kPer], child[kPer];
-    bool valid[kPer];
     uint32_t set = 0;
     if (a.sets) {  // set of this thread's first key: last s with set_key_off[s] <= i
       const uint64_t i0 = t * kTile + threadIdx.x;
@@ -164,6 +159,14 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
       }
       set = lo_s;
     }
+    // issued after every other load this tile waits for: vector memory
+    // counters retire in order, so a wait on a later load would wait for
+    // these too
+    uint64_t nk[kPer];
+    if (KIND == kKindU64 && t + gridDim.x < ntiles) load_u64(t + gridDim.x, nk);
+    uint64_t lo[kPer];
+    uint32_t meta[kPer], child[kPer];
+    bool valid[kPer];
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) {
       const uint64_t i = t * kTile + k * kThreads + threadIdx.x;
@@ -208,23 +211,27 @@ __global__ __launch_bounds__(kThreads) void k_split_level(SplitArgs a) {
   if (tile0 >= cnt) return;
   const uint64_t in0 = (uint64_t)parent * a.parent_cap;
   uint64_t lo[kPer];
-  uint32_t meta[kPer], child[kPer];
+  uint32_t m[kPer], meta[kPer], child[kPer];
   bool valid[kPer];
+  // all loads first (past the end: record tile0, which exists), then the
+  // child bins: pl / G_child = umulhi(pl, floor(2^32 / G_child) + 1) for
+  // pl < 2^16 and 2 <= G_child < 2^16 (the error stays below 1 / G_child)
 #pragma unroll
   for (uint32_t k = 0; k < kPer; k++) {
     const uint64_t i = tile0 + k * kThreads + threadIdx.x;
     valid[k] = i < cnt;
-    lo[k] = 0;
-    meta[k] = 0;
-    child[k] = 0;
-    if (valid[k]) {
-      lo[k] = a.in_lo[in0 + i];
-      const uint32_t m = a.in_meta[in0 + i];
-      const uint32_t pl = m >> 16;
-      const uint32_t c = pl / a.G_child;
-      meta[k] = ((pl - c * a.G_child) << 16) | (m & 0xFFFFu);
-      child[k] = c;
-    }
+    const uint64_t at = in0 + (valid[k] ? i : tile0);
+    lo[k] = a.in_lo[at];
+    m[k] = a.in_meta[at];
+  }
+  const uint32_t G = a.G_child;
+  const uint32_t inv = G > 1 ? (uint32_t)(0xFFFFFFFFu / G) + 1 : 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; k++) {
+    const uint32_t pl = m[k] >> 16;
+    const uint32_t c = G > 1 ? __umulhi(pl, inv) : pl;
+    meta[k] = ((pl - c * G) << 16) | (m[k] & 0xFFFFu);
+    child[k] = valid[k] ? c : 0;
   }
   route_tile(s, lo, meta, child, valid, (uint64_t)parent * a.fanout, (uint64_t)a.G_child * a.cap, a.child_count,
              a.out_lo, a.out_meta, a.status);
