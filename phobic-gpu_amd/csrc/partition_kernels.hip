@@ -159,6 +159,14 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
       }
       set = lo_s;
     }
+    // first key offset of the next set, and the set's partition range
+    uint64_t set_end = ~0ull;
+    uint32_t pb = 0, pn = a.np;
+    if (a.sets) {
+      if (set + 1 < a.sets) set_end = a.set_key_off[set + 1];
+      pb = a.set_part_base[set];
+      pn = a.set_part_base[set + 1] - pb;
+    }
     // issued after every other load this tile waits for: vector memory
     // counters retire in order, so a wait on a later load would wait for
     // these too
@@ -177,13 +185,15 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
       if (valid[k]) {
         const U128 h = KIND == kKindU64 ? xxh3_128_u64(kv[k], a.seed) : hash_key<KIND>(a.src, i, a.seed);
         uint32_t p;
-        if (a.sets) {
-          while (set + 1 < a.sets && a.set_key_off[set + 1] <= i) set++;
-          const uint32_t pb = a.set_part_base[set];
-          p = pb + partition_of((uint32_t)h.lo, a.set_part_base[set + 1] - pb);
-        } else {
-          p = partition_of((uint32_t)h.lo, a.np);
+        if (a.sets && i >= set_end) {  // a later set (loads only at a boundary)
+          do {
+            set++;
+            set_end = set + 1 < a.sets ? a.set_key_off[set + 1] : ~0ull;
+          } while (i >= set_end);
+          pb = a.set_part_base[set];
+          pn = a.set_part_base[set + 1] - pb;
         }
+        p = pb + partition_of((uint32_t)h.lo, pn);
         const uint32_t b = bucket_of((uint32_t)(h.lo >> 32), fl);
         const uint32_t c = p / a.G;
         lo[k] = h.hi;

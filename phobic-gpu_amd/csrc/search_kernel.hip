@@ -352,7 +352,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   lds_sync();
 
   // 2a. bin starts: exclusive scan over t = 256 - size (largest first)
-  uint32_t m, r_big;  // non-empty buckets; those with more than 64 keys
+  uint32_t m, r_big, n_one;  // non-empty buckets; those with more than 64 keys; with one key
   {
     uint32_t v[4], loc = 0;
 #pragma unroll
@@ -364,6 +364,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     uint32_t run = incl - loc;
     m = __builtin_amdgcn_readlane(incl, 63);
     r_big = __builtin_amdgcn_readlane(run, (256 - 64) / 4);  // bins t < 192: sizes 65..256
+    n_one = __builtin_amdgcn_readlane(v[3], 63);              // bin t = 255: size 1
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       hist[4 * lane + j] = run;
@@ -451,6 +452,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   const uint32_t pilot_full = tm ? (uint32_t)(0x80000000ull / tm) - 1 : 0;
   const uint32_t* Elane = E + lane;
   uint32_t err = 0;
+  // packed slot of key kv for pilot pl: (hash(l1, hash(l2, pl)) >> 1) mod tm
+  auto slot = [&](uint64_t kv, uint32_t pl) {
+    const uint32_t h = mix32((uint32_t)kv, mix32((uint32_t)(kv >> 32), pl)) >> 1;
+    return pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
+  };
 
   const bool multi = !(a.mode & kSearchSinglePilot);
   // Lane groups of a bucket of s <= 64 keys: 2^lg lanes per pilot (lg >= 2),
@@ -459,33 +465,17 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   auto lg_of = [&](uint32_t sz) -> uint32_t {
     return multi && sz <= 32 ? (sz <= 4 ? 2u : 32u - __builtin_clz(sz - 1)) : 6u;
   };
-  // A bucket's keys in registers: above 64 keys, key k in lane k & 63 of
-  // chunk k >> 6; otherwise lane l holds key min(l mod 2^lg, sz - 1), so
-  // every lane group holds the bucket (lanes past sz repeat its last key).
-  auto load_keys = [&](uint32_t st, uint32_t sz, uint64_t (&dst)[4]) {
-#pragma unroll
-    for (int j = 0; j < 4; j++) dst[j] = 0;
-    if (sz == 0) return;
-    if (sz <= 64) {
-      const uint32_t k = lane & ((1u << lg_of(sz)) - 1);
-      dst[0] = sorted[st + min(k, sz - 1)];
-      return;
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t k = lane + 64 * j;
-      if (64 * j < sz && k < sz) dst[j] = sorted[st + k];
-    }
+  struct Bucket {
+    uint32_t b = 0, s = 0, st = 0;  // index, keys, start in the slice (wave-uniform)
   };
-  uint64_t cur[4];
-  uint32_t cs = 0, cst = 0, cb = 0;
-  if (m) {
-    const uint2 e = reinterpret_cast<const uint2*>(rt)[0];
-    cb = __builtin_amdgcn_readfirstlane(e.x) & 0xFFFFu;
-    cs = __builtin_amdgcn_readfirstlane(e.x) >> 16;
-    cst = __builtin_amdgcn_readfirstlane(e.y);
-  }
-  load_keys(cst, cs, cur);
+  auto bucket_at = [&](uint32_t rank) {
+    const uint2 e = reinterpret_cast<const uint2*>(rt)[rank];
+    Bucket k;
+    k.b = __builtin_amdgcn_readfirstlane(e.x) & 0xFFFFu;
+    k.s = __builtin_amdgcn_readfirstlane(e.x) >> 16;
+    k.st = __builtin_amdgcn_readfirstlane(e.y);
+    return k;
+  };
   uint32_t placed = 0, r = 0;
 
   // Buckets r < r_end whose keys fit in C registers per lane (C chunks of
@@ -497,37 +487,48 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     constexpr int C = decltype(chunks)::value;
     constexpr int H = decltype(subwin)::value;
     constexpr bool kPad = C == 1;
+    if (r >= r_end) return;
+    // A bucket's keys in registers: with C > 1 key k in lane k & 63 of
+    // chunk k >> 6; with C = 1 lane l holds key min(l mod 2^lg, s - 1), so
+    // every lane group holds the bucket (lanes past s repeat its last key).
+    auto load_keys = [&](const Bucket& k, uint32_t lg, uint64_t (&dst)[C]) {
+      if constexpr (C == 1) {
+        dst[0] = sorted[k.st + min(lane & ((1u << lg) - 1), k.s - 1)];
+      } else {
+#pragma unroll
+        for (int j = 0; j < C; j++) {
+          const uint32_t kk = lane + 64 * j;
+          dst[j] = kk < k.s ? sorted[k.st + kk] : 0;
+        }
+      }
+    };
     uint32_t vfull[H];  // first-pass offset masks for dlimit = tm
 #pragma unroll
     for (int h = 0; h < H; h++) vfull[h] = below_mask(2048 * h + 32 * lane, tm);
+    Bucket cur = bucket_at(r);
+    uint32_t clg = C == 1 ? lg_of(cur.s) : 6u;
+    uint64_t key[C];  // the current bucket's keys, in the lane-group layout
+    load_keys(cur, clg, key);
     for (; r < r_end; r++) {
       // prefetch the next bucket's keys
-      uint64_t nxt[4] = {0, 0, 0, 0};
-      uint32_t ns = 0, nst = 0, nb = 0;
-      if (r + 1 < m) {
-        const uint2 e = reinterpret_cast<const uint2*>(rt)[r + 1];
-        nb = __builtin_amdgcn_readfirstlane(e.x) & 0xFFFFu;
-        ns = __builtin_amdgcn_readfirstlane(e.x) >> 16;
-        nst = __builtin_amdgcn_readfirstlane(e.y);
+      Bucket nxt;
+      uint32_t nlg = 6;
+      uint64_t nkey[C];
+      if (r + 1 < r_end) {
+        nxt = bucket_at(r + 1);
+        nlg = C == 1 ? lg_of(nxt.s) : 6u;
+        load_keys(nxt, nlg, nkey);
       }
-      load_keys(nst, ns, nxt);
 
-      const uint32_t s = cs;
+      const uint32_t s = cur.s, cst = cur.st, cb = cur.b;
       // the d < 64 window test pays off for large buckets in a sparse table
       const bool win0 = s >= kWin0MinKeys && 2 * placed < tm;
-      const uint32_t lg = C == 1 ? lg_of(s) : 6u;
+      const uint32_t lg = clg;
       const uint32_t G = 64u >> lg;
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
-      uint64_t key[C];  // already in the lane-group layout
-#pragma unroll
-      for (int j = 0; j < C; j++) key[j] = cur[j];
       uint32_t pk[C];  // packed slots
       uint32_t pilot = 0, dsel = 0;
       bool dupchk = false;
-      auto slot = [&](uint64_t kv, uint32_t pl) {
-        const uint32_t h = mix32((uint32_t)kv, mix32((uint32_t)(kv >> 32), pl)) >> 1;
-        return pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
-      };
       for (;; pilot++) {
         uint32_t dlimit = tm;  // d < dlimit keeps V = d + tm * pilot below 2^31
         if (pilot > pilot_full) {
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
 #pragma unroll
               for (int j = 0; j < C; j++) {
                 const uint32_t k = lane + 64 * j;
-                if (k < s && k != k2 && cur[j] == kv) dup = true;
+                if (k < s && k != k2 && key[j] == kv) dup = true;
               }
             }
             if (__any(dup)) {
@@ -639,11 +640,10 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       if (lane == 0) pil[cb] = dsel + tm * psel;
       placed += s;
       lds_sync();
+      cur = nxt;
+      clg = nlg;
 #pragma unroll
-      for (int j = 0; j < 4; j++) cur[j] = nxt[j];
-      cs = ns;
-      cst = nst;
-      cb = nb;
+      for (int j = 0; j < C; j++) key[j] = nkey[j];
     }
   };
   // processing order is size-descending: buckets above 64 keys come first.
@@ -652,11 +652,48 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
   using I4 = std::integral_constant<int, 4>;
+  // Buckets of one key come last.  Their pilot is always 0 (a free slot
+  // exists while keys are left, and slots of one key cannot coincide), so
+  // only the first free offset from the key's slot is searched: the slots of
+  // 64 of them are hashed at once, then each is scanned and marked in turn.
+  auto single_keys = [&](auto subwin) {
+    constexpr int H = decltype(subwin)::value;
+    uint32_t vfull[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) vfull[h] = below_mask(2048 * h + 32 * lane, tm);
+    for (; r < m; r += 64) {
+      const uint32_t cnt = min(64u, m - r);
+      uint32_t pk[1] = {0}, bi = 0;
+      if (lane < cnt) {
+        const uint2 e = reinterpret_cast<const uint2*>(rt)[r + lane];
+        bi = e.x & 0xFFFFu;
+        pk[0] = slot(sorted[e.y], 0);
+      }
+      for (uint32_t i = 0; i < cnt; i++) {
+        uint32_t dsel = 0;
+        scan_offsets<1, H>(Elane, pk, i, 1, tm, true, vfull, dsel);  // finds one: a slot is free
+        uint32_t q = unpack_slot(__builtin_amdgcn_readlane(pk[0], i)) + dsel;
+        if (q >= tm) q -= tm;
+        const uint32_t qc = q + lane * tm;  // the copies of slot q in E, one lane each
+        if (lane < 3 && qc < L) atomicOr(&E[qc >> 5], 1u << (qc & 31));
+        if (lane == 0) pil[__builtin_amdgcn_readlane(bi, i)] = dsel;
+        lds_sync();
+      }
+    }
+  };
+  const uint32_t r_one = m - n_one;
   search_buckets(I4{}, I1{}, min(r_big, m));
   if (!err) {
-    if (tm <= 2048 || (a.mode & kSearchNarrow)) search_buckets(I1{}, I1{}, m);
-    else if (tm <= 4096) search_buckets(I1{}, I2{}, m);
-    else search_buckets(I1{}, I4{}, m);
+    if (tm <= 2048 || (a.mode & kSearchNarrow)) {
+      search_buckets(I1{}, I1{}, r_one);
+      if (!err) single_keys(I1{});
+    } else if (tm <= 4096) {
+      search_buckets(I1{}, I2{}, r_one);
+      if (!err) single_keys(I2{});
+    } else {
+      search_buckets(I1{}, I4{}, r_one);
+      if (!err) single_keys(I4{});
+    }
   }
   if (err && lane == 0) atomicOr(a.status, 1u << err);
   lds_sync();
