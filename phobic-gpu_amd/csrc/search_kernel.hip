@@ -509,6 +509,9 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     uint32_t clg = C == 1 ? lg_of(cur.s) : 6u;
     uint64_t key[C];  // the current bucket's keys, in the lane-group layout
     load_keys(cur, clg, key);
+    // in before the loop, so that inside it only the next bucket's keys
+    // are ever in flight (a wait for older loads then leaves them alone)
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0) (gfx9 encoding)
     for (; r < r_end; r++) {
       // prefetch the next bucket's keys
       Bucket nxt;
