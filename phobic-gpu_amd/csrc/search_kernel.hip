@@ -531,8 +531,41 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
       uint32_t pk[C];  // packed slots
       uint32_t pilot = 0, dsel = 0;
-      bool dupchk = false;
-      for (;; pilot++) {
+      bool dupchk = false, done = false;
+      if constexpr (C == 1) {
+        // The common case, with less control flow: pilots whose offsets all
+        // keep V below 2^31 and that are below the duplicate check.  The
+        // bucket has at least two keys (one-key buckets are placed apart).
+        const uint32_t fast_end = min(kDupCheckPilot, pilot_full + 1);
+        for (; pilot < fast_end; pilot++) {
+          const uint32_t gi = pilot & (G - 1);
+          if (gi == 0) pk[0] = slot(key[0], pilot + g);
+          bool found = false;
+          if (win0) {
+            const uint64_t t0 = g == gi ? window64(E, pk[0]) : 0;
+            const uint64_t free0 = ~(((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0)) &
+                                   low_mask(tm);
+            if (free0) {
+              dsel = (uint32_t)__builtin_ctzll(free0);
+              found = true;
+            }
+          }
+          if (!found && !scan_offsets<1, H>(Elane, pk, gi << lg, s, tm, true, vfull, dsel)) continue;
+          bool c = false;
+          if (g == gi && kk < s) {
+            const uint32_t msk = 1u << (pk[0] & 31);
+            c = (atomicOr(&coll[pk[0] >> 7], msk) & msk) != 0;
+          }
+          lds_sync();
+          if (g == gi && kk < s) coll[pk[0] >> 7] = 0;
+          lds_sync();
+          if (!__any(c)) {
+            done = true;
+            break;
+          }
+        }
+      }
+      for (; !done; pilot++) {
         uint32_t dlimit = tm;  // d < dlimit keeps V = d + tm * pilot below 2^31
         if (pilot > pilot_full) {
           const uint64_t vbase = (uint64_t)tm * pilot;
