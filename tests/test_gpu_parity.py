@@ -131,6 +131,42 @@ def test_duplicate_keys_error(ctx):
     assert e.value.status == 7
 
 
+@pytest.mark.parametrize("size", [65, 150, 200])
+def test_large_bucket_matches_oracle(ctx, size):
+    """One bucket far above 64 keys (the chunked collision path): pilots and
+    offsets equal the oracle's."""
+    hk = po.hash_u64(po.splitmix64(11 + size, 6000), 0)
+    hk[:size, 0] = 5
+    hk[:size, 1] = 0xF0000000  # a late bucket: few keys of its own
+    r = ctx.build_keys(hk, ph.make_config(7.0, 2048))
+    st, pil, off = po.build_keys(hk, 7.0, 2048)
+    assert st == 0
+    assert np.array_equal(r.offsets, off) and np.array_equal(r.pilots, pil)
+
+
+def test_pilot_overflow_error(ctx):
+    """250 keys in one bucket of a ~2300-slot table need about 10^6 pilots:
+    V = d + tm * pilot passes 2^31 first, which both report."""
+    hk = po.hash_u64(po.splitmix64(261, 6000), 0)
+    hk[:250, 0] = 5
+    hk[:250, 1] = 0xF0000000
+    with pytest.raises(ph.PhobicError) as e:
+        ctx.build_keys(hk, ph.make_config(7.0, 2048))
+    assert e.value.status == 8
+    assert po.build_keys(hk, 7.0, 2048)[0] == 8
+
+
+def test_duplicate_keys_in_large_bucket_error(ctx):
+    hk = po.hash_u64(po.splitmix64(12, 6000), 0)
+    hk[:100, 0] = 5
+    hk[:100, 1] = 12345
+    hk[40, 2:] = hk[7, 2:]
+    with pytest.raises(ph.PhobicError) as e:
+        ctx.build_keys(hk, ph.make_config(7.0, 2048))
+    assert e.value.status == 7
+    assert po.build_keys(hk, 7.0, 2048)[0] == 7
+
+
 def test_bucket_overflow_error(ctx):
     hk = po.hash_u64(po.splitmix64(10, 4000), 0)
     hk[:300, 0] = 5
