@@ -132,7 +132,9 @@ def bench_batch(args) -> int:
     n = int(offs[-1])
     cfg = ph.make_config(7.0, 2048, 0)
     ctx = ph.Context(0)
-    tuning, _ = ctx.autotune(cfg, args.autotune or 8_000_000)  # launch shapes for this configuration
+    # launch shapes for this configuration, timed on a sample large enough
+    # to fill the GPU the way the batch does (a few 1e4 partitions)
+    tuning, _ = ctx.autotune(cfg, args.autotune or 64_000_000)
     stream = hip_rt.Stream()
     dk = hip_rt.DeviceBuffer(n * 8)
     for s in range(args.sets):

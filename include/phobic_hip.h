@@ -227,7 +227,9 @@ int phobic_server_destroy(phobic_server* server);
  * max_fanout, on the partitioning stages' device time) and then the search
  * shapes {1,2,4,8} x {0,1,2,3} (on the search stage's time) on
  * `sample_keys` device-generated keys built with `cfg`, keeps the fastest
- * on the context and returns it.  Results do not depend on the tuning. */
+ * on the context and returns it.  Results do not depend on the tuning.  The
+ * sample should span a few 1e4 partitions (tens of millions of keys): on
+ * fewer the search stage underfills the GPU and favours wide workgroups. */
 enum {
   PHOBIC_SEARCH_SINGLE_PILOT = 1, /* one pilot per hash round for every bucket */
   PHOBIC_SEARCH_NARROW = 2        /* scan passes of 2048 offsets for every table size */
