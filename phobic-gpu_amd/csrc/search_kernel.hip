@@ -61,6 +61,17 @@ __host__ __device__ inline uint32_t lds_words(uint32_t tcap, uint32_t B) {
   return (3 * B + union_words(tcap, B) + 1) & ~1u;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b mod 8).  This renumbers them so that each XCD takes a contiguous range
+// of [0, n): consecutive partitions then share an L2, which merges their
+// pilot writes (the interleaved layout p + b*np puts 16 partitions' pilots
+// in one 64-byte line) instead of sending partial lines from several L2s.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+  const uint32_t q = n / kXcds, rem = n % kXcds, x = b % kXcds, i = b / kXcds;
+  return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + i;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const uint32_t l = __lane_id();
   return l == 0 ? 0ull : (~0ull >> (64 - l));
@@ -296,7 +307,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   uint32_t* bcnt = cp;
   uint32_t* pil = cp;
 
-  const uint32_t p = blockIdx.x * kWavesPerGroup + wave;
+  const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * kWavesPerGroup + wave;
   if (p >= a.np) return;
   const uint32_t ps = __builtin_amdgcn_readfirstlane(a.pcount[p]);
   const uint64_t base = (uint64_t)p * cap;
