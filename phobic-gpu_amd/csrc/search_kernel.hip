@@ -238,6 +238,36 @@ __device__ __forceinline__ void or_bucket(const char* eb, const uint32_t (&pk)[C
   }
 }
 
+// One pass over the offsets [0, 2048 H) with masks vfull (a table of at
+// most 2048 H slots): the smallest offset at which the s slots of lane
+// group gbase (C = 1) are all free, or ~0u.
+template <int H>
+__device__ __forceinline__ uint32_t scan_pass(const uint32_t* Elane, uint32_t pk, uint32_t gbase, uint32_t s,
+                                              const uint32_t (&vfull)[H]) {
+  uint32_t acc[H];
+#pragma unroll
+  for (int h = 0; h < H; h++) acc[h] = 0;
+  const uint32_t pka[1] = {pk};
+  or_bucket<1, H>(reinterpret_cast<const char*>(Elane), pka, gbase, s, acc);
+  uint32_t fr[H];
+  uint64_t any[H], all = 0;
+#pragma unroll
+  for (int h = 0; h < H; h++) {
+    fr[h] = ~acc[h] & vfull[h];
+    any[h] = __ballot(fr[h] != 0);
+    all |= any[h];
+  }
+  if (!all) return ~0u;
+#pragma unroll
+  for (int h = 0; h < H; h++) {
+    if (any[h]) {
+      const uint32_t l0 = (uint32_t)__builtin_ctzll(any[h]);
+      return 2048 * h + 32 * l0 + (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(fr[h], l0));
+    }
+  }
+  return ~0u;
+}
+
 // Smallest offset d < dlimit at which the s slots of the bucket (held in pk
 // as or_bucket reads them) are all free, or false.  One pass covers
 // 2048 * H offsets: lane l ORs the 32-bit windows of every slot at
@@ -547,21 +577,10 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         // The common case, with less control flow: pilots whose offsets all
         // keep V below 2^31 and that are below the duplicate check.  The
         // bucket has at least two keys (one-key buckets are placed apart).
-        const uint32_t fast_end = min(kDupCheckPilot, pilot_full + 1);
-        for (; pilot < fast_end; pilot++) {
-          const uint32_t gi = pilot & (G - 1);
-          if (gi == 0) pk[0] = slot(key[0], pilot + g);
-          bool found = false;
-          if (win0) {
-            const uint64_t t0 = g == gi ? window64(E, pk[0]) : 0;
-            const uint64_t free0 = ~(((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0)) &
-                                   low_mask(tm);
-            if (free0) {
-              dsel = (uint32_t)__builtin_ctzll(free0);
-              found = true;
-            }
-          }
-          if (!found && !scan_offsets<1, H>(Elane, pk, gi << lg, s, tm, true, vfull, dsel)) continue;
+        // (one pass must cover the table: not in the narrow mode above 2048)
+        const uint32_t fast_end = tm <= 2048 * H ? min(kDupCheckPilot, pilot_full + 1) : 0;
+        // pairwise distinctness of pilot group gi's slots
+        auto distinct = [&](uint32_t gi) {
           bool c = false;
           if (g == gi && kk < s) {
             const uint32_t msk = 1u << (pk[0] & 31);
@@ -570,9 +589,34 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
           lds_sync();
           if (g == gi && kk < s) coll[pk[0] >> 7] = 0;
           lds_sync();
-          if (!__any(c)) {
-            done = true;
-            break;
+          return !__any(c);
+        };
+        if (win0) {
+          for (; pilot < fast_end; pilot++) {
+            const uint32_t gi = pilot & (G - 1);
+            if (gi == 0) pk[0] = slot(key[0], pilot + g);
+            const uint64_t t0 = g == gi ? window64(E, pk[0]) : 0;
+            const uint64_t free0 = ~(((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0)) &
+                                   low_mask(tm);
+            uint32_t d = free0 ? (uint32_t)__builtin_ctzll(free0) : scan_pass<H>(Elane, pk[0], gi << lg, s, vfull);
+            if (d == ~0u) continue;
+            if (distinct(gi)) {
+              dsel = d;
+              done = true;
+              break;
+            }
+          }
+        } else {
+          for (; pilot < fast_end; pilot++) {
+            const uint32_t gi = pilot & (G - 1);
+            if (gi == 0) pk[0] = slot(key[0], pilot + g);
+            const uint32_t d = scan_pass<H>(Elane, pk[0], gi << lg, s, vfull);
+            if (d == ~0u) continue;
+            if (distinct(gi)) {
+              dsel = d;
+              done = true;
+              break;
+            }
           }
         }
       }
