@@ -469,6 +469,10 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       m[j] = i < ps ? a.rec_meta[base + i] : 0xFFFFFFFFu;
       v[j] = i < ps ? a.rec_lower[base + i] : 0;
     }
+    // All of the batch's loads, waited for at once: the vector memory
+    // counter also counts stores, so a wait for a later load would
+    // otherwise hold each store until the one before it completed.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0) (gfx9 encoding)
 #pragma unroll
     for (uint32_t j = 0; j < kLoadBatch; j++) {
       if (m[j] != 0xFFFFFFFFu) {
