@@ -22,8 +22,15 @@
 // 2048, reads the 32-bit window of every key (consecutive words across the
 // lanes, so no bank conflicts) and ORs them; the lowest free bit of the
 // lowest lane is the smallest valid offset, which is the reference's answer.
-// Large buckets in a sparse table first try offsets 0..63 with one 64-bit
-// window per key and a DPP OR-reduction, which nearly always succeeds there.
+// Buckets of at least 6 keys in a table at most half full first try offsets
+// 0..63 with one 64-bit window per key and a DPP OR-reduction, which nearly
+// always succeeds there.
+//
+// Buckets come in three classes, in processing order: above 64 keys (their
+// chunks of 64 enter the collision bitmap one at a time, so most pilots are
+// dropped before any scan), 2..64 keys (lane groups hash several pilots per
+// round; pilots below 1024 run in a loop without the rare checks), and one
+// key (pilot 0 always; 64 slots hashed at once, then scanned one by one).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
