@@ -44,6 +44,8 @@ def test_c1_bit_exact_vs_golden(ctx):
     (5_000, 3.0, 64, 0, 7),
     (300_000, 8.0, 5000, 0, 8),      # tables above 4096 slots: four sub-windows per lane
     (200_000, 4.0, 9000, 0, 9),      # tables above 8192 slots: several scan passes
+    (300_000, 1.0, 2048, 0, 10),     # lambda 1: B = P buckets per partition, mostly one key
+    (300_000, 10.0, 2048, 0, 11),    # lambda 10: 205 buckets, the first ones well above 64 keys
 ])
 def test_u64_bit_exact_vs_oracle(ctx, n, lam, P, seed, kseed):
     keys = po.splitmix64(kseed, n)
