@@ -53,9 +53,9 @@ __host__ __device__ inline uint32_t subwindows(uint32_t tm) { return tm <= 2048 
 // (2*tm - 2) / 32 + 64 * subwindows(tm).
 __host__ __device__ inline uint32_t e_words(uint32_t tcap) { return (2 * tcap) / 32 + 64 * subwindows(tcap) + 8; }
 
-// LDS of one partition, in 32-bit words (an even count, so that every
-// wave's rank table is 8-byte aligned):
-//   rt[2B]  processing rank r -> {bucket | size << 16, start in the slice}
+// LDS of one partition, in 32-bit words:
+//   rt[B]   processing rank r -> bucket | size << 16 (a bucket's start in the
+//           slice is the sum of the sizes before it: the keys placed so far)
 //   cp[B]   steps 1-3: keys per bucket (LDS atomics); steps 5-6: pilot per bucket
 //   U       steps 1-2: hist[256], bucket count per t = 256 - size
 //           steps 3-4: bpos[B], next free index of each bucket in the slice
@@ -65,7 +65,7 @@ __host__ __device__ inline uint32_t union_words(uint32_t tcap, uint32_t B) {
   return max(u, max(256u, B));
 }
 __host__ __device__ inline uint32_t lds_words(uint32_t tcap, uint32_t B) {
-  return (3 * B + union_words(tcap, B) + 1) & ~1u;
+  return 2 * B + union_words(tcap, B);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
@@ -334,8 +334,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t cap = a.cap, B = a.B, tcap = a.tcap;
   const uint32_t EW = e_words(tcap), CW = (tcap + 31) / 32 + 1;
-  uint32_t* rt = smem + wave * lds_words(tcap, B);  // 2 words per rank
-  uint32_t* cp = rt + 2 * B;   // counts, then pilots, per bucket index
+  uint32_t* rt = smem + wave * lds_words(tcap, B);  // 1 word per rank
+  uint32_t* cp = rt + B;   // counts, then pilots, per bucket index
   uint32_t* U = cp + B;
   uint32_t* hist = U;          // steps 1-2
   uint32_t* bpos = U;          // steps 3-4
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t t = 256 - v;
       const uint32_t bv = hist[t];
       if (mine) {
-        rt[2 * (bv + __popcll(same & lanemask_lt()))] = (uint32_t)idx;
+        rt[bv + __popcll(same & lanemask_lt())] = (uint32_t)idx;
         act = false;
       }
       lds_sync();
@@ -451,13 +451,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     uint32_t carry = 0;
     for (uint32_t r0 = 0; r0 < m; r0 += 64) {
       const uint32_t r = r0 + lane;
-      const uint32_t b = r < m ? rt[2 * r] : 0;
+      const uint32_t b = r < m ? rt[r] : 0;
       const uint32_t s = r < m ? bcnt[b] : 0;
       const uint32_t incl = wave_incl_scan(s);
       if (r < m) {
         const uint32_t st = carry + incl - s;
-        rt[2 * r] = b | s << 16;
-        rt[2 * r + 1] = st;
+        rt[r] = b | s << 16;
         bpos[b] = st;
       }
       carry += __builtin_amdgcn_readlane(incl, 63);
@@ -520,12 +519,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   struct Bucket {
     uint32_t b = 0, s = 0, st = 0;  // index, keys, start in the slice (wave-uniform)
   };
-  auto bucket_at = [&](uint32_t rank) {
-    const uint2 e = reinterpret_cast<const uint2*>(rt)[rank];
+  auto bucket_at = [&](uint32_t rank, uint32_t start) {
+    const uint32_t e = __builtin_amdgcn_readfirstlane(rt[rank]);
     Bucket k;
-    k.b = __builtin_amdgcn_readfirstlane(e.x) & 0xFFFFu;
-    k.s = __builtin_amdgcn_readfirstlane(e.x) >> 16;
-    k.st = __builtin_amdgcn_readfirstlane(e.y);
+    k.b = e & 0xFFFFu;
+    k.s = e >> 16;
+    k.st = start;
     return k;
   };
   uint32_t placed = 0, r = 0;
@@ -557,7 +556,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     uint32_t vfull[H];  // first-pass offset masks for dlimit = tm
 #pragma unroll
     for (int h = 0; h < H; h++) vfull[h] = below_mask(2048 * h + 32 * lane, tm);
-    Bucket cur = bucket_at(r);
+    Bucket cur = bucket_at(r, placed);
     uint32_t clg = C == 1 ? lg_of(cur.s) : 6u;
     uint64_t key[C];  // the current bucket's keys, in the lane-group layout
     load_keys(cur, clg, key);
@@ -570,7 +569,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       uint32_t nlg = 6;
       uint64_t nkey[C];
       if (r + 1 < r_end) {
-        nxt = bucket_at(r + 1);
+        nxt = bucket_at(r + 1, placed + cur.s);
         nlg = C == 1 ? lg_of(nxt.s) : 6u;
         load_keys(nxt, nlg, nkey);
       }
@@ -763,13 +762,13 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     uint32_t vfull[H];
 #pragma unroll
     for (int h = 0; h < H; h++) vfull[h] = below_mask(2048 * h + 32 * lane, tm);
+    const uint32_t r0 = r, st0 = placed;  // one key each: rank r starts at st0 + r - r0
     for (; r < m; r += 64) {
       const uint32_t cnt = min(64u, m - r);
       uint32_t pk[1] = {0}, bi = 0;
       if (lane < cnt) {
-        const uint2 e = reinterpret_cast<const uint2*>(rt)[r + lane];
-        bi = e.x & 0xFFFFu;
-        pk[0] = slot(sorted[e.y], 0);
+        bi = rt[r + lane] & 0xFFFFu;
+        pk[0] = slot(sorted[st0 + (r - r0) + lane], 0);
       }
       for (uint32_t i = 0; i < cnt; i++) {
         uint32_t dsel = 0;
