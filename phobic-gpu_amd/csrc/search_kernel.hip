@@ -60,12 +60,30 @@ __host__ __device__ inline uint32_t e_words(uint32_t tcap) { return (2 * tcap) /
 //   U       steps 1-2: hist[256], bucket count per t = 256 - size
 //           steps 3-4: bpos[B], next free index of each bucket in the slice
 //           steps 5-6: E[e_words], coll[tcap / 32 + 1] (collision check)
-__host__ __device__ inline uint32_t union_words(uint32_t tcap, uint32_t B) {
-  const uint32_t u = e_words(tcap) + (tcap + 31) / 32 + 1;
-  return max(u, max(256u, B));
+// Word offsets of the regions inside one partition's LDS, used both by the
+// kernel's pointers and by the size the launch reserves.
+struct SearchLayout {
+  uint32_t rt, cp, U, coll, words;
+};
+__host__ __device__ inline SearchLayout search_layout(uint32_t tcap, uint32_t B) {
+  SearchLayout l;
+  l.rt = 0;
+  l.cp = l.rt + B;
+  l.U = l.cp + B;
+  l.coll = l.U + e_words(tcap);
+  // the union holds E + coll, or hist, or bpos
+  l.words = l.U + max(e_words(tcap) + (tcap + 31) / 32 + 1, max(256u, B));
+  return l;
 }
-__host__ __device__ inline uint32_t lds_words(uint32_t tcap, uint32_t B) {
-  return 2 * B + union_words(tcap, B);
+__host__ __device__ inline uint32_t lds_words(uint32_t tcap, uint32_t B) { return search_layout(tcap, B).words; }
+
+// The kernel forms its region pointers as rt, rt + B, rt + 2B, rt + 2B +
+// e_words(tcap) (this form compiles to the faster code); this checks that
+// they are search_layout's.
+inline bool layout_matches_kernel(uint32_t tcap, uint32_t B) {
+  const SearchLayout l = search_layout(tcap, B);
+  return l.rt == 0 && l.cp == B && l.U == 2 * B && l.coll == 2 * B + e_words(tcap) &&
+         l.words >= l.coll + (tcap + 31) / 32 + 1 && l.words >= l.U + max(256u, B);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
@@ -334,6 +352,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t cap = a.cap, B = a.B, tcap = a.tcap;
   const uint32_t EW = e_words(tcap), CW = (tcap + 31) / 32 + 1;
+  // the offsets of search_layout (layout_matches_kernel checks them on the
+  // host before every launch)
   uint32_t* rt = smem + wave * lds_words(tcap, B);  // 1 word per rank
   uint32_t* cp = rt + B;   // counts, then pilots, per bucket index
   uint32_t* U = cp + B;
@@ -849,6 +869,7 @@ size_t search_lds_bytes(uint32_t tcap, uint32_t B, uint32_t waves) { return (siz
 
 hipError_t launch_partition_search(const SearchArgs& a, uint32_t waves, hipStream_t s) {
   if (a.np == 0) return hipSuccess;
+  if (!layout_matches_kernel(a.tcap, a.B)) return hipErrorInvalidValue;
   const uint32_t groups = (a.np + waves - 1) / waves;
   const size_t lds = search_lds_bytes(a.tcap, a.B, waves);
   switch (waves) {
