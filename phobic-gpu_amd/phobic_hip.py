@@ -247,8 +247,12 @@ class BuildResult:
 
 
 def device_outputs(pilots, offsets, remap=None, remap_offsets=None) -> Outputs:
-    """phobic_outputs from device buffers (hip_rt.DeviceBuffer or addresses)."""
-    return Outputs(_ptr(pilots), _ptr(offsets), _ptr(remap) or None, _ptr(remap_offsets) or None)
+    """phobic_outputs from device buffers (hip_rt.DeviceBuffer or addresses).
+    The returned object keeps the buffer objects alive, so buffers created in
+    the call are not freed while a build still writes to them."""
+    out = Outputs(_ptr(pilots), _ptr(offsets), _ptr(remap) or None, _ptr(remap_offsets) or None)
+    out._keep = (pilots, offsets, remap, remap_offsets)
+    return out
 
 
 class Context:
