@@ -54,8 +54,9 @@ __host__ __device__ inline uint32_t subwindows(uint32_t tm) { return tm <= 2048 
 __host__ __device__ inline uint32_t e_words(uint32_t tcap) { return (2 * tcap) / 32 + 64 * subwindows(tcap) + 8; }
 
 // LDS of one partition, in 32-bit words:
-//   rt[B]   processing rank r -> bucket | size << 16 (a bucket's start in the
-//           slice is the sum of the sizes before it: the keys placed so far)
+//   rt[B]   processing rank r -> bucket | size << 16 | lane-group log2 << 25
+//           (a bucket's start in the slice is the sum of the sizes before
+//           it: the keys placed so far)
 //   cp[B]   steps 1-3: keys per bucket (LDS atomics); steps 5-6: pilot per bucket
 //   U       steps 1-2: hist[256], bucket count per t = 256 - size
 //           steps 3-4: bpos[B], next free index of each bucket in the slice
@@ -466,7 +467,16 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   }
   lds_sync();
 
-  // 3. bucket start offsets: the rank table and each bucket's cursor
+  // Lane groups of a bucket of s <= 64 keys: 2^lg lanes per pilot (lg >= 2),
+  // 64 / 2^lg pilots per hash round; one group (lg = 6) for s > 32 or with
+  // single-pilot rounds.
+  const bool multi = !(a.mode & kSearchSinglePilot);
+  auto lg_of = [&](uint32_t sz) -> uint32_t {
+    return multi && sz <= 32 ? (sz <= 4 ? 2u : 32u - __builtin_clz(sz - 1)) : 6u;
+  };
+
+  // 3. bucket start offsets: the rank table (bucket | size << 16 | lg << 25)
+  // and each bucket's cursor
   {
     uint32_t carry = 0;
     for (uint32_t r0 = 0; r0 < m; r0 += 64) {
@@ -476,7 +486,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t incl = wave_incl_scan(s);
       if (r < m) {
         const uint32_t st = carry + incl - s;
-        rt[r] = b | s << 16;
+        rt[r] = b | s << 16 | lg_of(s) << 25;
         bpos[b] = st;
       }
       carry += __builtin_amdgcn_readlane(incl, 63);
@@ -529,21 +539,15 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     return pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
   };
 
-  const bool multi = !(a.mode & kSearchSinglePilot);
-  // Lane groups of a bucket of s <= 64 keys: 2^lg lanes per pilot (lg >= 2),
-  // 64 / 2^lg pilots per hash round; one group (lg = 6) for s > 32 or with
-  // single-pilot rounds.
-  auto lg_of = [&](uint32_t sz) -> uint32_t {
-    return multi && sz <= 32 ? (sz <= 4 ? 2u : 32u - __builtin_clz(sz - 1)) : 6u;
-  };
   struct Bucket {
-    uint32_t b = 0, s = 0, st = 0;  // index, keys, start in the slice (wave-uniform)
+    uint32_t b = 0, s = 0, lg = 6, st = 0;  // index, keys, lane-group log2, start in the slice (wave-uniform)
   };
   auto bucket_at = [&](uint32_t rank, uint32_t start) {
     const uint32_t e = __builtin_amdgcn_readfirstlane(rt[rank]);
     Bucket k;
     k.b = e & 0xFFFFu;
-    k.s = e >> 16;
+    k.s = (e >> 16) & 0x1FFu;
+    k.lg = e >> 25;
     k.st = start;
     return k;
   };
@@ -577,7 +581,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
 #pragma unroll
     for (int h = 0; h < H; h++) vfull[h] = below_mask(2048 * h + 32 * lane, tm);
     Bucket cur = bucket_at(r, placed);
-    uint32_t clg = C == 1 ? lg_of(cur.s) : 6u;
+    uint32_t clg = C == 1 ? cur.lg : 6u;
     uint64_t key[C];  // the current bucket's keys, in the lane-group layout
     load_keys(cur, clg, key);
     // in before the loop, so that inside it only the next bucket's keys
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       uint64_t nkey[C];
       if (r + 1 < r_end) {
         nxt = bucket_at(r + 1, placed + cur.s);
-        nlg = C == 1 ? lg_of(nxt.s) : 6u;
+        nlg = C == 1 ? nxt.lg : 6u;
         load_keys(nxt, nlg, nkey);
       }
 
@@ -602,24 +606,38 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
       uint32_t pk[C];  // packed slots
       uint32_t pilot = 0, dsel = 0;
-      bool dupchk = false, done = false;
+      bool dupchk = false, done = false;  // done: placed and marked by the lean loop
       if constexpr (C == 1) {
         // The common case, with less control flow: pilots whose offsets all
         // keep V below 2^31 and that are below the duplicate check.  The
         // bucket has at least two keys (one-key buckets are placed apart).
         // (one pass must cover the table: not in the narrow mode above 2048)
         const uint32_t fast_end = tm <= 2048 * H ? min(kDupCheckPilot, pilot_full + 1) : 0;
-        // pairwise distinctness of pilot group gi's slots
-        auto distinct = [&](uint32_t gi) {
+        // Marks pilot group gi's slots at offset d in every copy of E and
+        // tells whether they are pairwise distinct.  The scan found them all
+        // free, so a bit that is already set means two of the bucket's own
+        // slots coincide: the marks are then taken back.
+        auto mark_distinct = [&](uint32_t gi, uint32_t d) {
+          const bool mine = g == gi && kk < s;
+          uint32_t q = unpack_slot(pk[0]) + d;
+          if (q >= tm) q -= tm;
+          const uint32_t q1 = q + tm;  // both below L = 2 tm + 128
+          const uint32_t m0 = 1u << (q & 31), m1 = 1u << (q1 & 31);
           bool c = false;
-          if (g == gi && kk < s) {
-            const uint32_t msk = 1u << (pk[0] & 31);
-            c = (atomicOr(&coll[pk[0] >> 7], msk) & msk) != 0;
+          if (mine) {
+            c = (atomicOr(&E[q >> 5], m0) & m0) != 0;
+            atomicOr(&E[q1 >> 5], m1);
+            for (uint32_t x = q1 + tm; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
           }
           lds_sync();
-          if (g == gi && kk < s) coll[pk[0] >> 7] = 0;
+          if (!__any(c)) return true;
+          if (mine) {
+            atomicAnd(&E[q >> 5], ~m0);
+            atomicAnd(&E[q1 >> 5], ~m1);
+            for (uint32_t x = q1 + tm; x < L; x += tm) atomicAnd(&E[x >> 5], ~(1u << (x & 31)));
+          }
           lds_sync();
-          return !__any(c);
+          return false;
         };
         if (win0) {
           for (; pilot < fast_end; pilot++) {
@@ -630,7 +648,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
                                    low_mask(tm);
             uint32_t d = free0 ? (uint32_t)__builtin_ctzll(free0) : scan_pass<H>(Elane, pk[0], gi << lg, s, vfull);
             if (d == ~0u) continue;
-            if (distinct(gi)) {
+            if (mark_distinct(gi, d)) {
               dsel = d;
               done = true;
               break;
@@ -642,7 +660,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
             if (gi == 0) pk[0] = slot(key[0], pilot + g);
             const uint32_t d = scan_pass<H>(Elane, pk[0], gi << lg, s, vfull);
             if (d == ~0u) continue;
-            if (distinct(gi)) {
+            if (mark_distinct(gi, d)) {
               dsel = d;
               done = true;
               break;
@@ -745,11 +763,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       }
       if (err) return;
       const uint32_t gsel = pilot & (G - 1), psel = pilot;
-      // mark the bucket's slots in every copy of E
+      // mark the bucket's slots in every copy of E (the lean loop has)
 #pragma unroll
       for (int j = 0; j < C; j++) {
         const uint32_t k = kk + 64 * j;
-        if (g == gsel && 64 * j < s && k < s) {
+        if (!done && g == gsel && 64 * j < s && k < s) {
           uint32_t q = unpack_slot(pk[j]) + dsel;
           if (q >= tm) q -= tm;
           const uint32_t q1 = q + tm;  // both below L = 2 tm + 128
