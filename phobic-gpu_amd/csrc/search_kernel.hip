@@ -177,7 +177,11 @@ __device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t M, uint32_t d) 
 // wrapped past 2^32 - 2d.
 __device__ __forceinline__ uint32_t mod_f32(uint32_t a, float rd, uint32_t d) {
   const uint32_t q = (uint32_t)(__uint2float_rn(a) * rd);
-  const uint32_t r = a - __umul24(q, d);
+  // the 24x24-bit product as one full-rate instruction (the compiler
+  // otherwise emits a mask and a quarter-rate v_mul_lo_u32 in some loops)
+  uint32_t qd;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(qd) : "s"(d), "v"(q));
+  const uint32_t r = a - qd;
   return min(min(r, r + d), r - d);
 }
 
