@@ -36,10 +36,12 @@ struct TileLds {
   uint64_t lo[kTile];
   uint32_t meta[kTile];
   uint8_t child[kTile];
+  // per child bin: records, then (once wave 0 has read them) lim: the
+  // tile's record d of the bin goes to output index base + d for d < lim,
+  // records from lim on are past the bin's capacity
   uint32_t hist[kMaxFanout];
   uint32_t lstart[kMaxFanout];
-  uint32_t gbase[kMaxFanout];
-  uint32_t wsum[kThreads / 64];
+  uint64_t base[kMaxFanout];
 };
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -52,14 +54,15 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-// Routes the block's tile (record k of thread t valid when valid[k]) to
-// child bins child_first + child[k] of capacity child_cap.  Wave 0 scans
-// the tile's bin histogram and takes the bins' global ranges with one
-// atomic each; the scatter into LDS overlaps those atomics.
+// Routes the block's tile (record k of thread t valid when valid[k],
+// `total` valid records) to child bins child_first + child[k] of capacity
+// child_cap.  Wave 0 scans the tile's bin histogram and takes the bins'
+// global ranges with one atomic each; the scatter into LDS overlaps those
+// atomics.
 template <uint32_t kPer>
 __device__ __forceinline__ void route_tile(TileLds<kPer>& s, const uint64_t (&lo)[kPer], const uint32_t (&meta)[kPer],
                                            const uint32_t (&child)[kPer], const bool (&valid)[kPer],
-                                           uint64_t child_first, uint64_t child_cap,
+                                           uint32_t total, uint64_t child_first, uint64_t child_cap,
                                            uint32_t* __restrict__ child_count, uint64_t* __restrict__ out_lo,
                                            uint32_t* __restrict__ out_meta, uint32_t* __restrict__ status) {
   static_assert(kMaxFanout == 128, "wave 0 scans two bins per lane");
@@ -71,14 +74,15 @@ __device__ __forceinline__ void route_tile(TileLds<kPer>& s, const uint64_t (&lo
   for (uint32_t k = 0; k < kPer; k++)
     if (valid[k]) rank[k] = atomicAdd(&s.hist[child[k]], 1u);
   __syncthreads();
-  uint32_t g0 = 0, g1 = 0;
+  uint32_t g0 = 0, g1 = 0, v0 = 0, v1 = 0, l0 = 0;
   if (tid < 64) {  // bins 2 tid and 2 tid + 1 (empty beyond the fan-out)
-    const uint32_t v0 = s.hist[2 * tid], v1 = s.hist[2 * tid + 1];
+    v0 = s.hist[2 * tid];
+    v1 = s.hist[2 * tid + 1];
     const uint32_t pair = v0 + v1;
     const uint32_t incl = wave_incl_scan(pair);
-    s.lstart[2 * tid] = incl - pair;
-    s.lstart[2 * tid + 1] = incl - pair + v0;
-    if (tid == 63) s.wsum[0] = incl;  // records in the tile
+    l0 = incl - pair;
+    s.lstart[2 * tid] = l0;
+    s.lstart[2 * tid + 1] = l0 + v0;
     if (v0) g0 = atomicAdd(&child_count[child_first + 2 * tid], v0);
     if (v1) g1 = atomicAdd(&child_count[child_first + 2 * tid + 1], v1);
   }
@@ -93,17 +97,21 @@ __device__ __forceinline__ void route_tile(TileLds<kPer>& s, const uint64_t (&lo
     }
   }
   if (tid < 64) {
-    s.gbase[2 * tid] = g0;
-    s.gbase[2 * tid + 1] = g1;
+    // record d of bin c goes to bin slot g_c + (d - lstart_c), inside the
+    // bin while that is below child_cap
+    auto bin = [&](uint32_t c, uint32_t g, uint32_t v, uint32_t l) {
+      s.base[c] = (child_first + c) * child_cap + g - l;
+      s.hist[c] = l + (uint32_t)min((uint64_t)v, child_cap > g ? child_cap - g : 0);
+    };
+    bin(2 * tid, g0, v0, l0);
+    bin(2 * tid + 1, g1, v1, l0 + v0);
   }
   __syncthreads();
-  const uint32_t total = s.wsum[0];
   bool over = false;
   for (uint32_t d = tid; d < total; d += kThreads) {
     const uint32_t c = s.child[d];
-    const uint64_t pos = (uint64_t)s.gbase[c] + (d - s.lstart[c]);
-    if (pos < child_cap) {
-      const uint64_t at = (child_first + c) * child_cap + pos;
+    if (d < s.hist[c]) {
+      const uint64_t at = s.base[c] + d;
       out_lo[at] = s.lo[d];
       out_meta[at] = s.meta[d];
     } else {
@@ -203,7 +211,9 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
         child[k] = c;
       }
     }
-    route_tile(s, lo, meta, child, valid, 0, (uint64_t)a.G * a.cap, a.child_count, a.out_lo, a.out_meta, a.status);
+    const uint32_t total = (uint32_t)min((uint64_t)kTile, a.n - t * kTile);  // valid records: a prefix
+    route_tile(s, lo, meta, child, valid, total, 0, (uint64_t)a.G * a.cap, a.child_count, a.out_lo, a.out_meta,
+               a.status);
     if (KIND == kKindU64) {
 #pragma unroll
       for (uint32_t k = 0; k < kPer; k++) kv[k] = nk[k];
@@ -245,7 +255,8 @@ __global__ __launch_bounds__(kThreads) void k_split_level(SplitArgs a) {
     meta[k] = ((pl - c * G) << 16) | (m[k] & 0xFFFFu);
     child[k] = valid[k] ? c : 0;
   }
-  route_tile(s, lo, meta, child, valid, (uint64_t)parent * a.fanout, (uint64_t)a.G_child * a.cap, a.child_count,
+  const uint32_t total = (uint32_t)min((uint64_t)kTile, cnt - tile0);  // valid records: a prefix
+  route_tile(s, lo, meta, child, valid, total, (uint64_t)parent * a.fanout, (uint64_t)a.G_child * a.cap, a.child_count,
              a.out_lo, a.out_meta, a.status);
 }
 
