@@ -47,9 +47,19 @@ struct U128 {
   uint64_t lo, hi;
 };
 
-__device__ __forceinline__ U128 mul128(uint64_t a, uint64_t b) { return {a * b, __umul64hi(a, b)}; }
+// The 128-bit product from four 32x32->64 multiply-adds (v_mad_u64_u32),
+// each partial product formed once.  No sum overflows: t and u are at most
+// (2^32-1)^2 + 2^32-1 < 2^64, and the high word is the exact quotient.
+__device__ __forceinline__ U128 mul128(uint64_t a, uint64_t b) {
+  const uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32), bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
+  const uint64_t p0 = (uint64_t)al * bl;
+  const uint64_t t = (uint64_t)ah * bl + (p0 >> 32);
+  const uint64_t u = (uint64_t)al * bh + (uint32_t)t;
+  return {(u << 32) | (uint32_t)p0, (uint64_t)ah * bh + (t >> 32) + (u >> 32)};
+}
 __device__ __forceinline__ uint64_t fold64(uint64_t a, uint64_t b) {
-  return (a * b) ^ __umul64hi(a, b);
+  const U128 m = mul128(a, b);
+  return m.lo ^ m.hi;
 }
 __device__ __forceinline__ uint64_t avalanche3(uint64_t h) {
   h ^= h >> 37;

@@ -147,6 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
   for (uint32_t i = threadIdx.x; i < kFulcs; i += kThreads) fl[i] = a.fulcs[i];
   __syncthreads();
   const uint64_t ntiles = (a.n + kTile - 1) / kTile;
+  const float rG = 1.0f / (float)a.G;
   // u64 keys: the next tile's keys are loaded while this tile is routed
   uint64_t kv[kPer];
   auto load_u64 = [&](uint64_t t, uint64_t (&dst)[kPer]) {
@@ -205,9 +206,20 @@ __global__ __launch_bounds__(kThreads) void k_hash_route(HashRouteArgs a) {
         }
         p = pb + partition_of((uint32_t)h.lo, pn);
         const uint32_t b = bucket_of((uint32_t)(h.lo >> 32), fl);
-        const uint32_t c = p / a.G;
+        // level-1 bin c = p / G and r = p mod G: the f32 quotient estimate
+        // is within one of c (c < 2^8, relative error below 2^-21), G <
+        // 2^16, so q * G is a 24-bit product and one step corrects it
+        uint32_t c = (uint32_t)(__uint2float_rn(p) * rG);
+        int32_t r = (int32_t)(p - __umul24(c, a.G));
+        if (r < 0) {
+          c--;
+          r += (int32_t)a.G;
+        } else if (r >= (int32_t)a.G) {
+          c++;
+          r -= (int32_t)a.G;
+        }
         lo[k] = h.hi;
-        meta[k] = ((p - c * a.G) << 16) | b;
+        meta[k] = ((uint32_t)r << 16) | b;
         child[k] = c;
       }
     }
