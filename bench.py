@@ -296,6 +296,16 @@ def bench_serve(args) -> int:
     return 0 if out["failed"] == 0 else 1
 
 
+def apply_tuning_flags(ctx, args) -> None:
+    """The --search-mode / --search-waves / ... overrides, on top of ctx's tuning."""
+    overrides = {"search_mode": args.search_mode, "search_waves": args.search_waves, "route_per": args.route_per,
+                 "max_fanout": args.max_fanout, "split_per": args.split_per, "first_fanout": args.first_fanout}
+    if any(v is not None for v in overrides.values()):
+        t = ctx.tuning()
+        t.update({k: v for k, v in overrides.items() if v is not None})
+        ctx.set_tuning(**t)
+
+
 def bench_strings(args) -> int:
     """Plain build of n distinct printable strings, lengths uniform in
     [8, 64] (numpy seed 6), bytes generated in HBM, lambda 6, P 2048,
@@ -308,6 +318,7 @@ def bench_strings(args) -> int:
     cfg = ph.make_config(lam, P, 0, alpha)
     np_, B = ph.sizes(cfg, n)
     ctx = ph.Context(0)
+    apply_tuning_flags(ctx, args)
     stream = hip_rt.Stream()
     d_offs = hip_rt.DeviceBuffer.from_numpy(offs)
     d_bytes = hip_rt.DeviceBuffer(total_bytes)
@@ -446,12 +457,7 @@ def main() -> int:
     tuning = None
     if args.autotune:
         tuning, _ = ctx.autotune(cfg, args.autotune)
-    overrides = {"search_mode": args.search_mode, "search_waves": args.search_waves, "route_per": args.route_per,
-                 "max_fanout": args.max_fanout, "split_per": args.split_per, "first_fanout": args.first_fanout}
-    if any(v is not None for v in overrides.values()):
-        t = ctx.tuning()
-        t.update({k: v for k, v in overrides.items() if v is not None})
-        ctx.set_tuning(**t)
+    apply_tuning_flags(ctx, args)
     stream = hip_rt.Stream()
     dk = hip_rt.DeviceBuffer(n * 8)
     ph.gen_splitmix64_device(args.key_seed, n, dk, stream)
