@@ -13,6 +13,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/phobic_hip.h"
@@ -99,6 +100,10 @@ struct phobic_ctx {
   DevBuf pcount, lvl_count, scan_tot, rec_lo[2], rec_meta[2], fulcs, status;
   // staging for host-buffer entry points
   DevBuf in_a, in_b, out_pilots, out_offsets, out_remap, out_roff;
+  // two pinned host chunks for large host <-> device copies (allocated on
+  // first use) and the events that say when a chunk's copy has finished
+  void* pinned[2] = {};
+  hipEvent_t pin_ev[2] = {};
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
   phobic_tuning tuning{2, 16, kMaxFanout, 0, 12, 32};
@@ -427,6 +432,83 @@ bool outputs_ok(const phobic_outputs* out, const Plan& pl) {
 }
 
 // Host-buffer build: stage inputs, enqueue on the context stream, copy out.
+// Host <-> device copies of the host-buffer entry points.  Pageable memory
+// moves at about half the link rate, so copies of at least kStageMin bytes
+// go through two pinned chunks: while one chunk's DMA runs, the next chunk
+// is copied into (or out of) the other by kCopyThreads host threads.
+constexpr size_t kStageChunk = 64ull << 20;
+constexpr size_t kStageMin = 16ull << 20;
+constexpr unsigned kCopyThreads = 4;
+
+void par_memcpy(void* dst, const void* src, size_t n) {
+  const size_t per = ((n + kCopyThreads - 1) / kCopyThreads + 63) & ~size_t(63);
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < kCopyThreads; t++) {
+    const size_t lo = t * per;
+    if (lo >= n) break;
+    th.emplace_back([=] { std::memcpy((char*)dst + lo, (const char*)src + lo, std::min(per, n - lo)); });
+  }
+  std::memcpy(dst, src, std::min(per, n));
+  for (auto& x : th) x.join();
+}
+
+int ensure_pinned(phobic_ctx* ctx) {
+  for (int i = 0; i < 2; i++) {
+    if (!ctx->pinned[i]) TRY_HIP(hipHostMalloc(&ctx->pinned[i], kStageChunk, hipHostMallocDefault));
+    if (!ctx->pin_ev[i]) TRY_HIP(hipEventCreateWithFlags(&ctx->pin_ev[i], hipEventDisableTiming));
+  }
+  return PHOBIC_OK;
+}
+
+// dst (device) <- src (host), ordered on s (src stays valid until the
+// caller's final synchronisation)
+int copy_to_device(phobic_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes < kStageMin) {
+    TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return PHOBIC_OK;
+  }
+  TRY(ensure_pinned(ctx));
+  for (size_t off = 0, i = 0; off < bytes; off += kStageChunk, i++) {
+    const int b = (int)(i & 1);
+    const size_t len = std::min(kStageChunk, bytes - off);
+    TRY_HIP(hipEventSynchronize(ctx->pin_ev[b]));  // the chunk's previous DMA is done
+    par_memcpy(ctx->pinned[b], (const char*)src + off, len);
+    TRY_HIP(hipMemcpyAsync((char*)dst + off, ctx->pinned[b], len, hipMemcpyHostToDevice, s));
+    TRY_HIP(hipEventRecord(ctx->pin_ev[b], s));
+  }
+  return PHOBIC_OK;
+}
+
+// dst (host) <- src (device) after the work enqueued on s; synchronous
+int copy_to_host(phobic_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes < kStageMin) {
+    TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    return PHOBIC_OK;
+  }
+  TRY(ensure_pinned(ctx));
+  const size_t chunks = (bytes + kStageChunk - 1) / kStageChunk;
+  auto issue = [&](size_t i) -> int {
+    const size_t off = i * kStageChunk;
+    TRY_HIP(hipMemcpyAsync(ctx->pinned[i & 1], (const char*)src + off, std::min(kStageChunk, bytes - off),
+                           hipMemcpyDeviceToHost, s));
+    TRY_HIP(hipEventRecord(ctx->pin_ev[i & 1], s));
+    return PHOBIC_OK;
+  };
+  TRY_HIP(hipEventSynchronize(ctx->pin_ev[0]));
+  TRY_HIP(hipEventSynchronize(ctx->pin_ev[1]));
+  TRY(issue(0));
+  for (size_t i = 0; i < chunks; i++) {
+    // chunk i + 1 goes to the other pinned chunk, whose last reader (the
+    // copy-out of chunk i - 1) has finished
+    if (i + 1 < chunks) TRY(issue(i + 1));
+    TRY_HIP(hipEventSynchronize(ctx->pin_ev[i & 1]));
+    const size_t off = i * kStageChunk;
+    par_memcpy((char*)dst + off, ctx->pinned[i & 1], std::min(kStageChunk, bytes - off));
+  }
+  return PHOBIC_OK;
+}
+
 int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const void* b, size_t b_bytes,
                uint64_t n, const phobic_config* cfg, const phobic_outputs* out, phobic_stage_times* times) {
   if (!ctx) return PHOBIC_E_INVALID_CONFIG;
@@ -436,10 +518,10 @@ int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const v
   TRY(set_device(ctx));
   hipStream_t s = ctx->stream;
   TRY_HIP(ctx->in_a.ensure(a_bytes, s));
-  if (a_bytes) TRY_HIP(hipMemcpyAsync(ctx->in_a.ptr, a, a_bytes, hipMemcpyHostToDevice, s));
+  if (a_bytes) TRY(copy_to_device(ctx, ctx->in_a.ptr, a, a_bytes, s));
   if (b) {
     TRY_HIP(ctx->in_b.ensure(b_bytes, s));
-    TRY_HIP(hipMemcpyAsync(ctx->in_b.ptr, b, b_bytes, hipMemcpyHostToDevice, s));
+    TRY(copy_to_device(ctx, ctx->in_b.ptr, b, b_bytes, s));
   }
   const uint64_t NB = (uint64_t)pl.np * pl.B;
   const bool remap = pl.a_ppm < 1000000u;
@@ -462,14 +544,12 @@ int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const v
   TRY(enqueue_build(ctx, kind, src, cfg, pl, d_out, d_status, s));
   uint32_t bits = 0;
   TRY_HIP(hipMemcpyAsync(&bits, d_status, 4, hipMemcpyDeviceToHost, s));
-  if (NB) TRY_HIP(hipMemcpyAsync(out->pilots, ctx->out_pilots.ptr, NB * 4, hipMemcpyDeviceToHost, s));
-  TRY_HIP(hipMemcpyAsync(out->offsets, ctx->out_offsets.ptr, ((uint64_t)pl.np + 1) * 4, hipMemcpyDeviceToHost, s));
-  if (remap)
-    TRY_HIP(hipMemcpyAsync(out->remap_offsets, ctx->out_roff.ptr, ((uint64_t)pl.np + 1) * 4, hipMemcpyDeviceToHost, s));
+  if (NB) TRY(copy_to_host(ctx, out->pilots, ctx->out_pilots.ptr, NB * 4, s));
+  TRY(copy_to_host(ctx, out->offsets, ctx->out_offsets.ptr, ((uint64_t)pl.np + 1) * 4, s));
+  if (remap) TRY(copy_to_host(ctx, out->remap_offsets, ctx->out_roff.ptr, ((uint64_t)pl.np + 1) * 4, s));
   TRY_HIP(hipStreamSynchronize(s));
-  if (remap && out->remap_offsets[pl.np]) {
-    TRY_HIP(hipMemcpy(out->remap, ctx->out_remap.ptr, (uint64_t)out->remap_offsets[pl.np] * 4, hipMemcpyDeviceToHost));
-  }
+  if (remap && out->remap_offsets[pl.np])
+    TRY(copy_to_host(ctx, out->remap, ctx->out_remap.ptr, (uint64_t)out->remap_offsets[pl.np] * 4, s));
   TRY(fill_times(ctx, times));
   return status_from_bits(bits);
 }
@@ -722,6 +802,10 @@ int phobic_ctx_destroy(phobic_ctx* ctx) {
     b->release(ctx->stream);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   phobic::dev_trim();
+  for (int i = 0; i < 2; i++) {
+    if (ctx->pinned[i]) (void)hipHostFree(ctx->pinned[i]);
+    if (ctx->pin_ev[i]) (void)hipEventDestroy(ctx->pin_ev[i]);
+  }
   for (int i = 0; i <= kStages; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -226,6 +226,56 @@ def test_device_build_matches_host_build(ctx):
     assert np.array_equal(do.to_numpy(np.uint32), r.offsets)
 
 
+def test_large_host_build_matches_device_build(ctx):
+    """Host-buffer builds large enough that keys and pilots move through the
+    pinned staging chunks (several chunks in, one out) equal the device
+    build of the same keys, remap table included."""
+    n, lam, P, alpha = 40_000_000, 7.5, 2048, 0.99
+    cfg = ph.make_config(lam, P, 0, alpha)
+    np_, B = ph.sizes(cfg, n)
+    rcap = ph.remap_capacity(cfg, n)
+    assert n * 8 > 64 << 20 and np_ * B * 4 > 16 << 20  # staged both ways
+    dk = hip_rt.DeviceBuffer(n * 8)
+    ph.gen_splitmix64_device(9, n, dk)
+    dp, do, ds = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
+    drm, dro = hip_rt.DeviceBuffer(rcap * 4), hip_rt.DeviceBuffer((np_ + 1) * 4)
+    ctx.build_u64_device(dk, n, cfg, ph.device_outputs(dp, do, drm, dro), ds)
+    hip_rt.synchronize()
+    assert int(ds.to_numpy(np.uint32)[0]) == 0
+    keys = dk.to_numpy(np.uint64)
+    for _ in range(2):  # the second call reuses the pinned chunks
+        r = ctx.build_u64(keys, cfg)
+        assert np.array_equal(dp.to_numpy(np.uint32), r.pilots)
+        assert np.array_equal(do.to_numpy(np.uint32), r.offsets)
+        roff = dro.to_numpy(np.uint32)
+        assert np.array_equal(roff, r.remap_offsets)
+        assert np.array_equal(drm.to_numpy(np.uint32, int(roff[-1])), r.remap)
+
+
+def test_large_host_string_build_matches_device_build(ctx):
+    """Strings whose bytes and offsets both move through the staging chunks."""
+    rng = np.random.default_rng(21)
+    n = 3_000_000
+    lens = rng.integers(8, 65, n)
+    offs = np.zeros(n + 1, np.uint64)
+    offs[1:] = np.cumsum(lens)
+    assert int(offs[-1]) > 64 << 20 and (n + 1) * 8 > 16 << 20
+    d_offs = hip_rt.DeviceBuffer.from_numpy(offs)
+    d_bytes = hip_rt.DeviceBuffer(int(offs[-1]))
+    ph.gen_strings_device(79, n, d_offs, d_bytes)
+    hip_rt.synchronize()
+    data = d_bytes.to_numpy(np.uint8, int(offs[-1]))
+    cfg = ph.make_config(6.0, 2048)
+    np_, B = ph.sizes(cfg, n)
+    dp, do, ds = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
+    ctx.build_bytes_device(d_bytes, d_offs, n, cfg, ph.device_outputs(dp, do), ds)
+    hip_rt.synchronize()
+    assert int(ds.to_numpy(np.uint32)[0]) == 0
+    r = ctx.build_bytes(data, offs, cfg)
+    assert np.array_equal(dp.to_numpy(np.uint32), r.pilots)
+    assert np.array_equal(do.to_numpy(np.uint32), r.offsets)
+
+
 def test_device_bijection_100m(ctx):
     """100M distinct keys (splitmix64 seed 2), lambda 8, P 2500: every key maps
     to a distinct slot of [0, n), checked on the device."""
