@@ -29,8 +29,10 @@
 // Buckets come in three classes, in processing order: above 64 keys (their
 // chunks of 64 enter the collision bitmap one at a time, so most pilots are
 // dropped before any scan), 2..64 keys (lane groups hash several pilots per
-// round; pilots below 1024 run in a loop without the rare checks), and one
-// key (pilot 0 always; 64 slots hashed at once, then scanned one by one).
+// round; pilots below 1024 run in a loop without the rare checks, whose
+// accepted pilot marks E directly and learns from the returned bits whether
+// two of its slots coincide), and one key (pilot 0 always; 64 slots hashed
+// at once, then scanned one by one).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
