@@ -221,9 +221,9 @@ int phobic_server_destroy(phobic_server* server);
  * split_per: records per thread of a tile of levels 2..L (0 = route_per);
  * first_fanout: at most this many level-1 bins (0 = max_fanout); the later
  *   levels then take the larger share of the fan-out.
- * Defaults {2, 16, 128, 0, 12, 32}.
+ * Defaults {2, 16, 128, 0, 12, 48}.
  * phobic_autotune times the partitioning shapes {8,12,16} x {8,12,16} x
- * {0,16,32,64} x {64,128} (route_per x split_per x first_fanout x
+ * {0,16,32,48,64} x {64,128} (route_per x split_per x first_fanout x
  * max_fanout, on the partitioning stages' device time) and then the search
  * shapes {1,2,4,8} x {0,1,2,3} (on the search stage's time) on
  * `sample_keys` device-generated keys built with `cfg`, keeps the fastest

@@ -106,7 +106,7 @@ struct phobic_ctx {
   hipEvent_t pin_ev[2] = {};
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
-  phobic_tuning tuning{2, 16, kMaxFanout, 0, 12, 32};
+  phobic_tuning tuning{2, 16, kMaxFanout, 0, 12, 48};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
@@ -968,7 +968,7 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   float part_ms = 1e30f, search_ms = 1e30f;
   for (uint32_t per : {8u, 12u, 16u})
     for (uint32_t sper : {8u, 12u, 16u})
-      for (uint32_t f1 : {0u, 16u, 32u, 64u})
+      for (uint32_t f1 : {0u, 16u, 32u, 48u, 64u})
         for (uint32_t f : {64u, 128u}) {
           if (f1 > f) continue;
           phobic_tuning t = win;
