@@ -440,15 +440,23 @@ constexpr size_t kStageChunk = 64ull << 20;
 constexpr size_t kStageMin = 16ull << 20;
 constexpr unsigned kCopyThreads = 4;
 
+// Slice t of kCopyThreads goes to a thread of its own; slices whose thread
+// cannot be started (no exception may leave the C ABI) are copied here.
 void par_memcpy(void* dst, const void* src, size_t n) {
   const size_t per = ((n + kCopyThreads - 1) / kCopyThreads + 63) & ~size_t(63);
-  std::vector<std::thread> th;
-  for (unsigned t = 1; t < kCopyThreads; t++) {
+  auto slice = [=](unsigned t) {
     const size_t lo = t * per;
-    if (lo >= n) break;
-    th.emplace_back([=] { std::memcpy((char*)dst + lo, (const char*)src + lo, std::min(per, n - lo)); });
+    if (lo < n) std::memcpy((char*)dst + lo, (const char*)src + lo, std::min(per, n - lo));
+  };
+  std::vector<std::thread> th;
+  unsigned t = 1;
+  try {
+    th.reserve(kCopyThreads - 1);
+    for (; t < kCopyThreads && t * per < n; t++) th.emplace_back(slice, t);
+  } catch (...) {
   }
-  std::memcpy(dst, src, std::min(per, n));
+  for (unsigned u = t; u < kCopyThreads; u++) slice(u);
+  slice(0);
   for (auto& x : th) x.join();
 }
 
