@@ -651,7 +651,13 @@ int phobic_batch_create(phobic_ctx* ctx, const phobic_config* cfg, const uint64_
   if (alpha != 1.0) return PHOBIC_E_INVALID_CONFIG;  // batches use the reference load factor
   Plan one;
   TRY(validate(cfg, 0, &one));
-  std::vector<uint32_t> pb(sets + 1, 0);
+  // no exception may leave the C ABI: allocations that throw become E_OOM
+  std::vector<uint32_t> pb;
+  try {
+    pb.assign(sets + 1, 0);
+  } catch (...) {
+    return PHOBIC_E_OOM;
+  }
   uint64_t total_parts = 0;
   for (uint32_t s = 0; s < sets; s++) {
     if (set_offsets[s + 1] < set_offsets[s]) return PHOBIC_E_INVALID_CONFIG;
@@ -677,13 +683,18 @@ int phobic_batch_create(phobic_ctx* ctx, const phobic_config* cfg, const uint64_
   if (!b) return PHOBIC_E_OOM;
   b->ctx = ctx;
   b->cfg = *cfg;
-  if (cfg->fulcrums) {
-    b->fulcs_copy.assign(cfg->fulcrums, cfg->fulcrums + kFulcsTable);
-    b->cfg.fulcrums = b->fulcs_copy.data();
+  try {
+    if (cfg->fulcrums) {
+      b->fulcs_copy.assign(cfg->fulcrums, cfg->fulcrums + kFulcsTable);
+      b->cfg.fulcrums = b->fulcs_copy.data();
+    }
+    b->part_base = std::move(pb);
+  } catch (...) {
+    delete b;
+    return PHOBIC_E_OOM;
   }
   b->pl = pl;
   b->sets = sets;
-  b->part_base = pb;
   int st = set_device(ctx);
   hipStream_t s = ctx->stream;
   auto fail = [&](int code) {
@@ -699,7 +710,8 @@ int phobic_batch_create(phobic_ctx* ctx, const phobic_config* cfg, const uint64_
   if (e == hipSuccess) e = b->d_part_base.ensure((uint64_t)(sets + 1) * 4, s);
   if (e == hipSuccess) e = b->d_goff.ensure(((uint64_t)pl.np + 1) * 4, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_key_off.ptr, set_offsets, (uint64_t)(sets + 1) * 8, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(b->d_part_base.ptr, pb.data(), (uint64_t)(sets + 1) * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_part_base.ptr, b->part_base.data(), (uint64_t)(sets + 1) * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return fail(hip_status(e));
   st = prepare_build(ctx, &b->cfg, pl, s);

@@ -191,7 +191,10 @@ void worker(phobic_server* s, size_t idx, hipStream_t stream) {
       }
       s->completed++;
       if (st == PHOBIC_OK) s->keys_built += job->req.n;
-      s->latencies_ms.push_back(std::chrono::duration<double, std::milli>(job->t_done - job->t_submit).count());
+      try {  // a sample that cannot be stored is dropped from the statistics
+        s->latencies_ms.push_back(std::chrono::duration<double, std::milli>(job->t_done - job->t_submit).count());
+      } catch (...) {
+      }
       s->t_last = job->t_done;
       try_admit(s);
     }
@@ -220,7 +223,15 @@ int phobic_server_create(const phobic_server_config* cfg, phobic_server** out) {
     delete s;
     return PHOBIC_E_NO_DEVICE;
   }
-  s->slots.resize(cfg->slots);
+  // no exception may leave the C ABI: allocations and threads that fail
+  // become E_OOM, and the partly built server is torn down
+  try {
+    s->slots.resize(cfg->slots);
+  } catch (...) {
+    (void)hipStreamDestroy(s->free_stream);
+    delete s;
+    return PHOBIC_E_OOM;
+  }
   for (auto& sl : s->slots) {
     const int st = phobic_ctx_create(cfg->device, &sl.ctx);
     if (st != PHOBIC_OK) {
@@ -233,7 +244,12 @@ int phobic_server_create(const phobic_server_config* cfg, phobic_server** out) {
   }
   for (size_t i = 0; i < s->slots.size(); i++) {
     hipStream_t st = phobic::ctx_stream(s->slots[i].ctx);
-    s->slots[i].th = std::thread(worker, s, i, st);
+    try {
+      s->slots[i].th = std::thread(worker, s, i, st);
+    } catch (...) {
+      phobic_server_destroy(s);  // joins the workers already started
+      return PHOBIC_E_OOM;
+    }
   }
   *out = s;
   return PHOBIC_OK;
@@ -244,12 +260,17 @@ int phobic_server_submit(phobic_server* s, const phobic_request* req, uint64_t* 
   if (req->kind != PHOBIC_KEYS_U64 && req->kind != PHOBIC_KEYS_BYTES) return PHOBIC_E_INVALID_CONFIG;
   if (req->n && !req->keys) return PHOBIC_E_INVALID_CONFIG;
   if (req->kind == PHOBIC_KEYS_BYTES && !req->offsets) return PHOBIC_E_INVALID_CONFIG;
-  auto r = std::make_shared<Request>();
-  r->req = *req;
-  r->cfg = req->cfg;
-  if (req->cfg.fulcrums) {
-    r->fulcs.assign(req->cfg.fulcrums, req->cfg.fulcrums + 2048);
-    r->cfg.fulcrums = r->fulcs.data();
+  std::shared_ptr<Request> r;
+  try {
+    r = std::make_shared<Request>();
+    r->req = *req;
+    r->cfg = req->cfg;
+    if (req->cfg.fulcrums) {
+      r->fulcs.assign(req->cfg.fulcrums, req->cfg.fulcrums + 2048);
+      r->cfg.fulcrums = r->fulcs.data();
+    }
+  } catch (...) {
+    return PHOBIC_E_OOM;
   }
   int st = phobic_query_sizes(&r->cfg, req->n, &r->np, &r->B);
   if (st) return st;
@@ -267,13 +288,22 @@ int phobic_server_submit(phobic_server* s, const phobic_request* req, uint64_t* 
   }
   r->id = s->next_id++;
   r->t_submit = Clock::now();
+  try {  // both containers hold the request, or neither does
+    s->queue.push_back(r);
+  } catch (...) {
+    return PHOBIC_E_OOM;
+  }
+  try {
+    s->all[r->id] = r;
+  } catch (...) {
+    s->queue.pop_back();
+    return PHOBIC_E_OOM;
+  }
   if (!s->any) {
     s->t_first = r->t_submit;
     s->any = true;
   }
   s->submitted++;
-  s->all[r->id] = r;
-  s->queue.push_back(r);
   *id = r->id;
   try_admit(s);
   return PHOBIC_OK;
@@ -375,7 +405,12 @@ int phobic_server_stats_get(phobic_server* s, phobic_server_stats* out) {
   out->rejected = s->rejected;
   out->keys_built = s->keys_built;
   out->hbm_in_use = s->mem_in_use;
-  std::vector<double> v = s->latencies_ms;
+  std::vector<double> v;
+  try {
+    v = s->latencies_ms;
+  } catch (...) {
+    return PHOBIC_E_OOM;
+  }
   if (!v.empty()) {
     std::sort(v.begin(), v.end());
     auto pct = [&](double q) {
