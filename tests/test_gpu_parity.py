@@ -228,13 +228,13 @@ def test_device_build_matches_host_build(ctx):
 
 def test_large_host_build_matches_device_build(ctx):
     """Host-buffer builds large enough that keys and pilots move through the
-    pinned staging chunks (several chunks in, one out) equal the device
+    pinned staging chunks (16 chunks of 64 MB in, 2 out) equal the device
     build of the same keys, remap table included."""
-    n, lam, P, alpha = 40_000_000, 7.5, 2048, 0.99
+    n, lam, P, alpha = 130_000_000, 7.5, 2048, 0.99
     cfg = ph.make_config(lam, P, 0, alpha)
     np_, B = ph.sizes(cfg, n)
     rcap = ph.remap_capacity(cfg, n)
-    assert n * 8 > 64 << 20 and np_ * B * 4 > 16 << 20  # staged both ways
+    assert n * 8 > 15 * (64 << 20) and np_ * B * 4 > 64 << 20  # several chunks both ways
     dk = hip_rt.DeviceBuffer(n * 8)
     ph.gen_splitmix64_device(9, n, dk)
     dp, do, ds = hip_rt.DeviceBuffer(np_ * B * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
