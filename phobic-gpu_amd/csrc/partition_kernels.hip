@@ -23,6 +23,7 @@
 
 #include "device_hash.hpp"
 #include "kernels.hpp"
+#include "xcd.hpp"
 
 namespace phobic {
 namespace {
@@ -239,8 +240,11 @@ template <uint32_t kPer>
 __global__ __launch_bounds__(kThreads) void k_split_level(SplitArgs a) {
   constexpr uint32_t kTile = kThreads * kPer;
   __shared__ TileLds<kPer> s;
-  const uint32_t parent = blockIdx.x / a.slices;
-  const uint64_t tile0 = (uint64_t)(blockIdx.x % a.slices) * kTile;
+  // neighbouring tiles of a parent on one XCD: their runs in each child bin
+  // are adjacent, so the lines they share are merged in one L2
+  const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t parent = bid / a.slices;
+  const uint64_t tile0 = (uint64_t)(bid % a.slices) * kTile;
   const uint32_t cnt = min(a.parent_count[parent], (uint32_t)min((uint64_t)0xFFFFFFFFu, a.parent_cap));
   if (tile0 >= cnt) return;
   const uint64_t in0 = (uint64_t)parent * a.parent_cap;

@@ -40,6 +40,7 @@
 
 #include "device_hash.hpp"
 #include "kernels.hpp"
+#include "xcd.hpp"
 
 namespace phobic {
 namespace {
@@ -89,16 +90,6 @@ inline bool layout_matches_kernel(uint32_t tcap, uint32_t B) {
          l.words >= l.coll + (tcap + 31) / 32 + 1 && l.words >= l.U + max(256u, B);
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
-// b mod 8).  This renumbers them so that each XCD takes a contiguous range
-// of [0, n): consecutive partitions then share an L2, which merges their
-// pilot writes (the interleaved layout p + b*np puts 16 partitions' pilots
-// in one 64-byte line) instead of sending partial lines from several L2s.
-constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
-  const uint32_t q = n / kXcds, rem = n % kXcds, x = b % kXcds, i = b / kXcds;
-  return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + i;
-}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const uint32_t l = __lane_id();
@@ -371,6 +362,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   uint32_t* bcnt = cp;
   uint32_t* pil = cp;
 
+  // consecutive partitions on one XCD: the interleaved layout p + b*np puts
+  // 16 partitions' pilots in one 64-byte line
   const uint32_t p = xcd_block(blockIdx.x, gridDim.x) * kWavesPerGroup + wave;
   if (p >= a.np) return;
   const uint32_t ps = __builtin_amdgcn_readfirstlane(a.pcount[p]);
