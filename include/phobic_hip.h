@@ -221,7 +221,7 @@ int phobic_server_destroy(phobic_server* server);
  * split_per: records per thread of a tile of levels 2..L (0 = route_per);
  * first_fanout: at most this many level-1 bins (0 = max_fanout); the later
  *   levels then take the larger share of the fan-out.
- * Defaults {2, 16, 128, 0, 12, 48}.
+ * Defaults {1, 16, 128, 0, 12, 48}.
  * phobic_autotune times the partitioning shapes {8,12,16} x {8,12,16} x
  * {0,16,32,48,64} x {64,128} (route_per x split_per x first_fanout x
  * max_fanout, on the partitioning stages' device time) and then the search

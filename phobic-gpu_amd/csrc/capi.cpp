@@ -106,7 +106,7 @@ struct phobic_ctx {
   hipEvent_t pin_ev[2] = {};
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
-  phobic_tuning tuning{2, 16, kMaxFanout, 0, 12, 48};
+  phobic_tuning tuning{1, 16, kMaxFanout, 0, 12, 48};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
