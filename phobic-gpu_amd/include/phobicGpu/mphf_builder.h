@@ -82,12 +82,19 @@ class MPHFbuilder {
         if (!keys[i].empty()) std::memcpy(bytes.data() + offs[i], keys[i].data(), keys[i].size());
       timer.addLabel("input_packing");
       st = phobic_build_bytes(ctx, bytes.data(), offs.data(), n, &cfg, &out, &t);
-    } else if constexpr (std::is_same_v<keyType, std::pair<uint64_t, uint64_t>>) {
+    } else if constexpr (std::is_same_v<keyType, std::pair<uint64_t, uint64_t>> || std::is_same_v<keyType, Key>) {
+      // 16 bytes per key, hashed as the reference hashes a pair or a Key
+      // (XXH128 of the value's bytes, hasher.hpp:51-66)
+      static_assert(sizeof(Key) == 16, "Key is four 32-bit words");
       std::vector<uint64_t> offs(n + 1), words(2 * n);
       for (uint64_t i = 0; i < n; i++) {
         offs[i] = 16 * i;
-        words[2 * i] = keys[i].first;
-        words[2 * i + 1] = keys[i].second;
+        if constexpr (std::is_same_v<keyType, Key>) {
+          std::memcpy(&words[2 * i], &keys[i], 16);
+        } else {
+          words[2 * i] = keys[i].first;
+          words[2 * i + 1] = keys[i].second;
+        }
       }
       offs[n] = 16 * n;
       st = phobic_build_bytes(ctx, reinterpret_cast<const uint8_t*>(words.data()), offs.data(), n, &cfg, &out, &t);

@@ -1,6 +1,7 @@
-"""The C++ API end to end on the GPU: the compiled example builds an MPHF
-over string keys through MPHFbuilder (the C ABI underneath) and checks that
-it is a bijection onto [0, n), for both of the reference's typedefs."""
+"""The C++ API end to end on the GPU: the compiled examples build MPHFs
+through MPHFbuilder (the C ABI underneath) over string keys, for both of the
+reference's typedefs, and over every other key type the reference's hasher
+takes, and check that each is a bijection onto [0, n)."""
 import os
 import subprocess
 
@@ -10,6 +11,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXAMPLE = os.path.join(ROOT, "phobic-gpu_amd", "examples", "bin", "example")
+KEY_TYPES = os.path.join(ROOT, "phobic-gpu_amd", "examples", "bin", "key_types")
 
 
 @pytest.mark.parametrize("kind", ["fast", "small"])
@@ -22,3 +24,15 @@ def test_example_builds_a_bijection(kind):
     assert "valid for 300000" in r.stdout
     bpk = float(r.stdout.split("total_bits=")[1].split()[0])
     assert 1.5 < bpk < 4.0
+
+
+def test_every_key_type_builds_a_bijection():
+    """MPHFbuilder::build over uint64_t, std::pair<uint64_t, uint64_t> and Key
+    (hashed as the reference's xxhash does, hasher.hpp:28-66), and over
+    pre-hashed Key values with nohash; each MPHF is a bijection."""
+    if not os.path.exists(KEY_TYPES):
+        pytest.fail(f"{KEY_TYPES} not built (run __graft_entry__.build())")
+    r = subprocess.run([KEY_TYPES, "200000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for name in ("uint64_t", "pair<uint64_t,uint64_t>", "Key (xxhash)", "Key (nohash)"):
+        assert f"{name} valid for 200000" in r.stdout
