@@ -54,7 +54,13 @@ int main(int argc, char** argv) {
     ks[i] = Key((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32));
   }
   using NoHashMphf = MPHF<interleaved_encoder<compact>, diff_partition_encoder<compact>, nohash>;
-  bool ok = check<FastQueryMphf>("uint64_t", u);
+  // the reference's static initial hash (mphf.hpp:61): XXH3-128 of the 8
+  // bytes of 1 with seed 0 is 0xbdc94bce2eda264d_bc08dc21994df8a2
+  const Key h = FastQueryMphf::initialHash(uint64_t(1));
+  bool ok = h.partitioner == 0x994df8a2u && h.bucketer == 0xbc08dc21u && h.lower1 == 0x2eda264du &&
+            h.lower2 == 0xbdc94bceu;
+  std::cout << "initialHash " << (ok ? "matches" : "differs from") << " the known answer" << std::endl;
+  ok = check<FastQueryMphf>("uint64_t", u) && ok;
   ok = check<FastQueryMphf>("pair<uint64_t,uint64_t>", pr) && ok;
   ok = check<FastQueryMphf>("Key (xxhash)", ks) && ok;
   ok = check<NoHashMphf>("Key (nohash)", ks) && ok;

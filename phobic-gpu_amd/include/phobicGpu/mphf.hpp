@@ -133,9 +133,11 @@ class MPHF {
 
   constexpr static bool noHash() { return std::is_same_v<Hasher, nohash>; }
 
+  // static as in the reference (mphf.hpp:61); the seed is this build's
+  // extension, 0 being the reference's hash
   template <typename keyType>
-  inline Key initialHash(const keyType& keyRaw) const {
-    return Hasher::hash(keyRaw, seed);
+  static inline Key initialHash(const keyType& keyRaw, uint64_t hashSeed = 0) {
+    return Hasher::hash(keyRaw, hashSeed);
   }
 
   // pilots: interleaved p + b*np; offsets: np+1 exclusive (mphf.hpp:65-75)
@@ -153,7 +155,7 @@ class MPHF {
 
   template <typename keyType>
   inline uint32_t operator()(const keyType& keyRaw) const {
-    const Key key = initialHash(keyRaw);
+    const Key key = initialHash(keyRaw, seed);
     const uint64_t partition = (uint64_t(key.partitioner) * uint64_t(partitions)) >> 32;
     const uint64_t bucket = getBucket(key.bucketer);
     const uint32_t pilot = (uint32_t)pilots.access(partition, bucket);

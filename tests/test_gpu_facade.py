@@ -34,5 +34,6 @@ def test_every_key_type_builds_a_bijection():
         pytest.fail(f"{KEY_TYPES} not built (run __graft_entry__.build())")
     r = subprocess.run([KEY_TYPES, "200000"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert "initialHash matches the known answer" in r.stdout
     for name in ("uint64_t", "pair<uint64_t,uint64_t>", "Key (xxhash)", "Key (nohash)"):
         assert f"{name} valid for 200000" in r.stdout
