@@ -43,9 +43,20 @@ class MPHFbuilder {
   }
 
  public:
-  explicit MPHFbuilder(MPHFconfig config = MPHFconfig(), int device = 0) : config(config), device(device) {}
-  MPHFbuilder(const MPHFbuilder&) = delete;
-  MPHFbuilder& operator=(const MPHFbuilder&) = delete;
+  // implicit like the reference's (mphf_builder.h:36)
+  MPHFbuilder(MPHFconfig config = MPHFconfig(), int device = 0) : config(config), device(device) {}
+  // copies share the configuration; each opens its own device context on
+  // its first build
+  MPHFbuilder(const MPHFbuilder& o) : config(o.config), device(o.device) {}
+  MPHFbuilder& operator=(const MPHFbuilder& o) {
+    if (this != &o) {
+      if (ctx) phobic_ctx_destroy(ctx);
+      ctx = nullptr;
+      config = o.config;
+      device = o.device;
+    }
+    return *this;
+  }
   ~MPHFbuilder() {
     if (ctx) phobic_ctx_destroy(ctx);
   }
