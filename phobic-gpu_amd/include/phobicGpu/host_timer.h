@@ -1,6 +1,7 @@
 // host_timer.h -- labelled wall-clock splits (reference include/app/host_timer.h:7-31).
 #pragma once
 #include <chrono>
+#include <iostream>
 #include <string>
 #include <vector>
 
@@ -28,5 +29,14 @@ class HostTimer {
       last = l.time;
     }
     return res;
+  }
+  // one line per split: its own time, the time since reset, its name, each
+  // time divided by div (host_timer.cpp:20-28)
+  void printLabels(double div) const {
+    double last = 0;
+    for (const Label& l : labels) {
+      std::cout << (l.time - last) / div << " " << l.time / div << " " << l.name << std::endl;
+      last = l.time;
+    }
   }
 };
