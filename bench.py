@@ -430,7 +430,7 @@ def main() -> int:
                     help="sample keys for the launch-shape autotuner (0 = default shapes)")
     ap.add_argument("--search-mode", type=int, default=None, help="pilot-search variant mask (0..3)")
     ap.add_argument("--search-waves", type=int, default=None, help="partitions per search workgroup")
-    ap.add_argument("--route-per", type=int, default=None, help="records per thread of a routing tile (8 or 16)")
+    ap.add_argument("--route-per", type=int, default=None, help="records per thread of a level-1 routing tile (8, 12, 16 or 20)")
     ap.add_argument("--max-fanout", type=int, default=None, help="bins per partitioning level (16..128)")
     ap.add_argument("--split-per", type=int, default=None, help="records per thread of a split tile (0 = route-per)")
     ap.add_argument("--first-fanout", type=int, default=None, help="at most this many level-1 bins (0 = max-fanout)")

@@ -213,16 +213,17 @@ int phobic_server_destroy(phobic_server* server);
 
 /* ---- launch shapes and the autotuner ----
  * search_waves: partitions (one wave64 each) per search workgroup (1,2,4,8);
- * route_per: records per thread of a level-1 (hash and route) tile (8, 12
- *   or 16);
+ * route_per: records per thread of a level-1 (hash and route) tile of u64
+ *   or pre-hashed keys (8, 12, 16 or 20); byte strings always use 8;
  * max_fanout: bins per partitioning level (16..128);
  * search_mode: pilot-search variants, a mask of PHOBIC_SEARCH_* (0 = all
  *   of the fast paths);
- * split_per: records per thread of a tile of levels 2..L (0 = route_per);
+ * split_per: records per thread of a tile of levels 2..L (8, 12 or 16; 0 =
+ *   route_per, at most 16);
  * first_fanout: at most this many level-1 bins (0 = max_fanout); the later
  *   levels then take the larger share of the fan-out.
- * Defaults {1, 16, 128, 0, 12, 48}.
- * phobic_autotune times the partitioning shapes {8,12,16} x {8,12,16} x
+ * Defaults {1, 20, 128, 0, 12, 48}.
+ * phobic_autotune times the partitioning shapes {8,12,16,20} x {8,12,16} x
  * {0,16,32,48,64} x {64,128} (route_per x split_per x first_fanout x
  * max_fanout, on the partitioning stages' device time) and then the search
  * shapes {1,2,4,8} x {0,1,2,3} (on the search stage's time) on

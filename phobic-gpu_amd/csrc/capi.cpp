@@ -106,7 +106,7 @@ struct phobic_ctx {
   hipEvent_t pin_ev[2] = {};
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
-  phobic_tuning tuning{1, 16, kMaxFanout, 0, 12, 48};
+  phobic_tuning tuning{1, 20, kMaxFanout, 0, 12, 48};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
@@ -301,7 +301,8 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
                  const phobic_outputs& d_out, uint32_t* d_status, hipStream_t s, const BatchInfo& batch, bool timed) {
   const Plan pl = tuned(ctx, pl0);
   const uint32_t per = ctx->tuning.route_per;
-  const uint32_t split_per = ctx->tuning.split_per ? ctx->tuning.split_per : per;
+  // split tiles come in 8, 12 and 16 records per thread
+  const uint32_t split_per = ctx->tuning.split_per ? ctx->tuning.split_per : std::min(per, 16u);
   uint32_t* d_pilots = d_out.pilots;
   const bool remap = pl.a_ppm < 1000000u;
   const uint32_t np = pl.np, L = pl.L, f = pl.fanout;
@@ -931,7 +932,7 @@ int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* t) {
   if (!ctx || !t) return PHOBIC_E_INVALID_CONFIG;
   const uint32_t w = t->search_waves, per = t->route_per, f = t->max_fanout;
   const uint32_t sp = t->split_per, f1 = t->first_fanout;
-  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 12 || per == 16) || f < 16 || f > kMaxFanout ||
+  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 12 || per == 16 || per == 20) || f < 16 || f > kMaxFanout ||
       t->search_mode > 3 || !(sp == 0 || sp == 8 || sp == 12 || sp == 16) || (f1 != 0 && (f1 < 2 || f1 > f)))
     return PHOBIC_E_INVALID_CONFIG;
   ctx->tuning = *t;
@@ -986,7 +987,7 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   };
   phobic_tuning win = saved;
   float part_ms = 1e30f, search_ms = 1e30f;
-  for (uint32_t per : {8u, 12u, 16u})
+  for (uint32_t per : {8u, 12u, 16u, 20u})
     for (uint32_t sper : {8u, 12u, 16u})
       for (uint32_t f1 : {0u, 16u, 32u, 48u, 64u})
         for (uint32_t f : {64u, 128u}) {

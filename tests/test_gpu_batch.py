@@ -102,7 +102,7 @@ def test_tuning_does_not_change_results(ctx):
 def test_autotune_returns_valid_shape(ctx):
     saved = ctx.tuning()
     best, ms = ctx.autotune(ph.make_config(7.0, 2048), 2_000_000)
-    assert best["search_waves"] in (1, 2, 4, 8) and best["route_per"] in (8, 12, 16)
+    assert best["search_waves"] in (1, 2, 4, 8) and best["route_per"] in (8, 12, 16, 20)
     assert best["max_fanout"] in (64, 128) and best["search_mode"] in (0, 1, 2, 3) and ms > 0
     assert best["split_per"] in (8, 12, 16) and best["first_fanout"] in (0, 16, 32, 48, 64)
     assert ctx.tuning() == best
