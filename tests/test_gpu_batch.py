@@ -90,13 +90,36 @@ def test_tuning_does_not_change_results(ctx):
     try:
         shapes = [(1, 8, 32, 1, 0, 0), (2, 16, 64, 2, 0, 0), (8, 8, 128, 3, 0, 0), (4, 16, 16, 0, 0, 0),
                   (4, 16, 128, 3, 0, 0), (2, 12, 128, 0, 0, 0), (2, 12, 40, 1, 0, 0), (2, 16, 128, 0, 8, 32),
-                  (2, 16, 64, 0, 16, 16), (1, 8, 128, 0, 12, 2), (2, 16, 16, 0, 8, 3)]
+                  (2, 16, 64, 0, 16, 16), (1, 8, 128, 0, 12, 2), (2, 16, 16, 0, 8, 3),
+                  (1, 20, 128, 0, 12, 48), (2, 20, 64, 0, 0, 32)]  # split_per 0 with 20: split tiles of 16
         for shape in shapes:
             ctx.set_tuning(*shape)
             r = ctx.build_u64(keys, cfg)
             assert np.array_equal(r.pilots, ref.pilots) and np.array_equal(r.offsets, ref.offsets), shape
     finally:
         ctx.set_tuning(**saved)
+
+
+def test_tuning_defaults_and_validation():
+    """A new context starts with the documented shapes (include/phobic_hip.h);
+    out-of-range shapes are refused and leave the tuning unchanged."""
+    c = ph.Context(0)
+    try:
+        d = c.tuning()
+        assert d == {"search_waves": 1, "route_per": 20, "max_fanout": 128, "search_mode": 0, "split_per": 12,
+                     "first_fanout": 48}
+        for bad in ({"search_waves": 3}, {"route_per": 24}, {"route_per": 10}, {"split_per": 20},
+                    {"max_fanout": 129}, {"max_fanout": 8}, {"search_mode": 4}, {"first_fanout": 1},
+                    {"first_fanout": 200}):
+            t = dict(d)
+            t.update(bad)
+            with pytest.raises(ph.PhobicError):
+                c.set_tuning(**t)
+            assert c.tuning() == d, bad
+        c.set_tuning(2, 16, 64, 1, 8, 32)
+        assert c.tuning()["route_per"] == 16 and c.tuning()["split_per"] == 8
+    finally:
+        c.close()
 
 
 def test_autotune_returns_valid_shape(ctx):
