@@ -41,7 +41,8 @@ typedef enum {
   PHOBIC_E_DUPLICATE_KEYS = 7,   /* two keys hash to the same 128 bits */
   PHOBIC_E_PILOT_OVERFLOW = 8,   /* no pilot value below 2^31 fits */
   PHOBIC_E_BUSY = 9,             /* serving: request rejected (queue full) */
-  PHOBIC_E_NOT_FOUND = 10        /* serving: unknown request id */
+  PHOBIC_E_NOT_FOUND = 10,       /* serving: unknown request id */
+  PHOBIC_E_SCRATCH_LIMIT = 11    /* allocation-free build: needs more than its fixed scratch */
 } phobic_status;
 
 /* MPHFconfig (include/phobicGpu/mphf_config.h:16-65).  With buckets = 0 and
@@ -141,6 +142,27 @@ int phobic_build_bytes_device(phobic_ctx* ctx, const uint8_t* d_bytes, const uin
 int phobic_build_keys_device(phobic_ctx* ctx, const phobic_key* d_keys, uint64_t n,
                              const phobic_config* cfg, const phobic_outputs* d_out, uint32_t* d_status,
                              void* stream);
+
+/* ---- allocation-free build (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments, results and asynchrony as phobic_build_u64_device, but the
+ * build's scratch (partitioning records, bin counters, scan totals, the
+ * fulcrum table) lives in one fixed device array of
+ * PHOBIC_NOALLOC_SCRATCH_BYTES per device, which each device the library
+ * runs on holds until the process ends; the call makes no allocation.  A
+ * build needs about 24 * (P + P/2) bytes per partition (partitions rounded
+ * up to the routing's bin size) plus small tables: at P = 2048 up to about
+ * 1.8 million keys fit.  Larger builds return PHOBIC_E_SCRATCH_LIMIT before
+ * anything is enqueued.  Every call on a device shares that device's array
+ * and the array keeps the last call's contents (what a build needs zeroed
+ * it clears itself): calls that overlap in time on one device race and
+ * give wrong results, so serialise them (one stream, or wait between
+ * calls).  The context's stage timers record the build as for
+ * phobic_build_u64_device. */
+#define PHOBIC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int phobic_build_u64_device_noalloc(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t n, const phobic_config* cfg,
+                                    const phobic_outputs* d_out, uint32_t* d_status, void* stream);
 
 /* ---- batched builds: many independent u64 key sets, one configuration ----
  * set s = d_keys[set_offsets[s] .. set_offsets[s+1]) (set_offsets: host,

@@ -279,26 +279,67 @@ uint32_t search_waves(const phobic_ctx* ctx, const Plan& pl) {
   return w;
 }
 
+// Device scratch of one build: partitioning records (two ping-pong
+// buffers), bin counters, scan totals and the fulcrum table.
+struct Scratch {
+  uint32_t* pcount;
+  uint32_t* lvl_count;
+  uint32_t* scan_tot;
+  uint64_t* rec_lo[2];
+  uint32_t* rec_meta[2];
+  uint32_t* fulcs;
+};
+
+// Bytes of each scratch region for a (tuned) plan, in Scratch's order.
+struct ScratchSizes {
+  uint64_t pcount, lvl_count, scan_tot, rec_lo, rec_meta, fulcs;
+};
+ScratchSizes scratch_sizes(const Plan& pl) {
+  const Layout ly = layout_of(pl);
+  ScratchSizes z;
+  z.pcount = ((uint64_t)pl.np + pl.fanout) * 4;
+  z.lvl_count = (ly.cnt_total + 1) * 4;
+  z.scan_tot = (2ull * partition_scan_tiles(pl.np) + 2) * 4;
+  z.rec_lo = ly.slots * 8;
+  z.rec_meta = ly.slots * 4;
+  z.fulcs = kFulcsTable * 4;
+  return z;
+}
+
+Scratch ctx_scratch(const phobic_ctx* ctx) {
+  Scratch sc;
+  sc.pcount = ctx->pcount.as<uint32_t>();
+  sc.lvl_count = ctx->lvl_count.as<uint32_t>();
+  sc.scan_tot = ctx->scan_tot.as<uint32_t>();
+  for (int i = 0; i < 2; i++) {
+    sc.rec_lo[i] = ctx->rec_lo[i].as<uint64_t>();
+    sc.rec_meta[i] = ctx->rec_meta[i].as<uint32_t>();
+  }
+  sc.fulcs = ctx->fulcs.as<uint32_t>();
+  return sc;
+}
+
 // Allocations and the fulcrum upload: everything a build needs before its
 // kernels are enqueued (not capturable into a graph).
 int prepare_build(phobic_ctx* ctx, const phobic_config* cfg, const Plan& pl0, hipStream_t s) {
   const Plan pl = tuned(ctx, pl0);
   TRY(upload_fulcrums(ctx, cfg, s));
-  const Layout ly = layout_of(pl);
-  TRY_HIP(ctx->pcount.ensure(((uint64_t)pl.np + pl.fanout) * 4, s));
-  TRY_HIP(ctx->lvl_count.ensure((ly.cnt_total + 1) * 4, s));
-  TRY_HIP(ctx->scan_tot.ensure((2ull * partition_scan_tiles(pl.np) + 2) * 4, s));
+  const ScratchSizes z = scratch_sizes(pl);
+  TRY_HIP(ctx->pcount.ensure(z.pcount, s));
+  TRY_HIP(ctx->lvl_count.ensure(z.lvl_count, s));
+  TRY_HIP(ctx->scan_tot.ensure(z.scan_tot, s));
   for (int i = 0; i < 2; i++) {
-    TRY_HIP(ctx->rec_lo[i].ensure(ly.slots * 8, s));
-    TRY_HIP(ctx->rec_meta[i].ensure(ly.slots * 4, s));
+    TRY_HIP(ctx->rec_lo[i].ensure(z.rec_lo, s));
+    TRY_HIP(ctx->rec_meta[i].ensure(z.rec_meta, s));
   }
   return PHOBIC_OK;
 }
 
 // Kernel launches of one build (fills included, no allocation, no host
 // copies): capturable into a hipGraph when `timed` is false.
-int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_config* cfg, const Plan& pl0,
-                 const phobic_outputs& d_out, uint32_t* d_status, hipStream_t s, const BatchInfo& batch, bool timed) {
+int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& src, const phobic_config* cfg,
+                 const Plan& pl0, const phobic_outputs& d_out, uint32_t* d_status, hipStream_t s,
+                 const BatchInfo& batch, bool timed) {
   const Plan pl = tuned(ctx, pl0);
   const uint32_t per = ctx->tuning.route_per;
   // split tiles come in 8, 12 and 16 records per thread
@@ -309,14 +350,14 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
   const Layout ly = layout_of(pl);
   const uint32_t bins1 = ly.bins1;
   TRY_HIP(launch_fill_u32(d_status, 1, 0, s));
-  TRY_HIP(launch_fill_u32(ctx->pcount.as<uint32_t>(), (uint64_t)np + f, 0, s));
-  TRY_HIP(launch_fill_u32(ctx->lvl_count.as<uint32_t>(), ly.cnt_total, 0, s));
+  TRY_HIP(launch_fill_u32(sc.pcount, (uint64_t)np + f, 0, s));
+  TRY_HIP(launch_fill_u32(sc.lvl_count, ly.cnt_total, 0, s));
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[0], s));
 
   // level l writes buffer 0 when L - l is even, so level L lands in buffer 0
   auto out_buf = [&](uint32_t l) { return (int)((L - l) & 1); };
   auto counts = [&](uint32_t l) {
-    return l == L ? ctx->pcount.as<uint32_t>() : ctx->lvl_count.as<uint32_t>() + ly.cnt_off[l];
+    return l == L ? sc.pcount : sc.lvl_count + ly.cnt_off[l];
   };
   HashRouteArgs ha{};
   ha.src = src;
@@ -326,10 +367,10 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
   ha.cap = pl.cap;
   ha.G = (uint32_t)pl.G[1];
   ha.fanout = bins1;
-  ha.fulcs = ctx->fulcs.as<uint32_t>();
+  ha.fulcs = sc.fulcs;
   ha.child_count = counts(1);
-  ha.out_lo = ctx->rec_lo[out_buf(1)].as<uint64_t>();
-  ha.out_meta = ctx->rec_meta[out_buf(1)].as<uint32_t>();
+  ha.out_lo = sc.rec_lo[out_buf(1)];
+  ha.out_meta = sc.rec_meta[out_buf(1)];
   ha.status = d_status;
   ha.sets = batch.sets;
   ha.set_key_off = batch.set_key_off;
@@ -340,8 +381,8 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
   for (uint32_t l = 2; l <= L; l++) {
     SplitArgs sp{};
     const uint32_t parents = (uint32_t)((np + pl.G[l - 1] - 1) / pl.G[l - 1]);
-    sp.in_lo = ctx->rec_lo[out_buf(l - 1)].as<uint64_t>();
-    sp.in_meta = ctx->rec_meta[out_buf(l - 1)].as<uint32_t>();
+    sp.in_lo = sc.rec_lo[out_buf(l - 1)];
+    sp.in_meta = sc.rec_meta[out_buf(l - 1)];
     sp.parent_count = counts(l - 1);
     sp.parent_cap = pl.G[l - 1] * pl.cap;
     sp.slices = (uint32_t)((sp.parent_cap + route_tile_records(split_per) - 1) / route_tile_records(split_per));
@@ -349,29 +390,29 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
     sp.G_child = (uint32_t)pl.G[l];
     sp.cap = pl.cap;
     sp.child_count = counts(l);
-    sp.out_lo = ctx->rec_lo[out_buf(l)].as<uint64_t>();
-    sp.out_meta = ctx->rec_meta[out_buf(l)].as<uint32_t>();
+    sp.out_lo = sc.rec_lo[out_buf(l)];
+    sp.out_meta = sc.rec_meta[out_buf(l)];
     sp.status = d_status;
     TRY_HIP(launch_split_level(sp, parents, split_per, s));
   }
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[2], s));
 
-  uint32_t* totals = ctx->scan_tot.as<uint32_t>();
+  uint32_t* totals = sc.scan_tot;
   if (batch.sets) {  // batches build at alpha = 1
-    TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, 0, batch.global_offsets, nullptr, totals, s));
+    TRY_HIP(launch_partition_offsets(sc.pcount, np, 0, batch.global_offsets, nullptr, totals, s));
     TRY_HIP(launch_batch_offsets(batch.global_offsets, batch.set_part_base, batch.sets, np + batch.sets,
                                  d_out.offsets, s));
   } else {
-    TRY_HIP(launch_partition_offsets(ctx->pcount.as<uint32_t>(), np, remap ? pl.a_ppm : 0, d_out.offsets,
+    TRY_HIP(launch_partition_offsets(sc.pcount, np, remap ? pl.a_ppm : 0, d_out.offsets,
                                      remap ? d_out.remap_offsets : nullptr, totals, s));
   }
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[3], s));
 
   SearchArgs ra{};
-  ra.rec_lower = ctx->rec_lo[0].as<uint64_t>();
-  ra.rec_meta = ctx->rec_meta[0].as<uint32_t>();
-  ra.pcount = ctx->pcount.as<uint32_t>();
-  ra.sorted = ctx->rec_lo[1].as<uint64_t>();  // free once partitioning is done
+  ra.rec_lower = sc.rec_lo[0];
+  ra.rec_meta = sc.rec_meta[0];
+  ra.pcount = sc.pcount;
+  ra.sorted = sc.rec_lo[1];  // free once partitioning is done
   ra.np = np;
   ra.B = pl.B;
   ra.cap = pl.cap;
@@ -396,7 +437,7 @@ int launch_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_c
 int enqueue_build(phobic_ctx* ctx, int kind, const KeySource& src, const phobic_config* cfg, const Plan& pl,
                   const phobic_outputs& d_out, uint32_t* d_status, hipStream_t s) {
   TRY(prepare_build(ctx, cfg, pl, s));
-  return launch_build(ctx, kind, src, cfg, pl, d_out, d_status, s, BatchInfo{}, true);
+  return launch_build(ctx, ctx_scratch(ctx), kind, src, cfg, pl, d_out, d_status, s, BatchInfo{}, true);
 }
 
 int status_from_bits(uint32_t bits) {
@@ -577,6 +618,50 @@ int device_build(phobic_ctx* ctx, int kind, const KeySource& src, uint64_t n, co
 
 hipStream_t phobic::ctx_stream(phobic_ctx* ctx) { return ctx->stream; }
 
+int phobic::ctx_device(const phobic_ctx* ctx) { return ctx->device; }
+
+int phobic::build_u64_in_scratch(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t n, const phobic_config* cfg,
+                                 const phobic_outputs* d_out, uint32_t* d_status, hipStream_t s, void* base,
+                                 size_t bytes, uint32_t* fulcs_host) {
+  if (!ctx || !d_status || !base || !fulcs_host || (n && !d_keys)) return PHOBIC_E_INVALID_CONFIG;
+  Plan pl0;
+  TRY(validate(cfg, n, &pl0));
+  if (!outputs_ok(d_out, pl0)) return PHOBIC_E_INVALID_CONFIG;
+  // regions at 256-byte boundaries; the fulcrum table always first, so the
+  // copy recorded in fulcs_host stays valid from one call to the next
+  const ScratchSizes z = scratch_sizes(tuned(ctx, pl0));
+  uint64_t end = 0;
+  auto take = [&](uint64_t sz) {
+    const uint64_t at = end;
+    end += (sz + 255) & ~uint64_t(255);
+    return at;
+  };
+  const uint64_t o_f = take(z.fulcs), o_pc = take(z.pcount), o_lc = take(z.lvl_count), o_st = take(z.scan_tot);
+  const uint64_t o_lo0 = take(z.rec_lo), o_lo1 = take(z.rec_lo), o_m0 = take(z.rec_meta), o_m1 = take(z.rec_meta);
+  if (end > bytes) return PHOBIC_E_SCRATCH_LIMIT;
+  TRY(set_device(ctx));
+  char* b = static_cast<char*>(base);
+  Scratch sc;
+  sc.fulcs = reinterpret_cast<uint32_t*>(b + o_f);
+  sc.pcount = reinterpret_cast<uint32_t*>(b + o_pc);
+  sc.lvl_count = reinterpret_cast<uint32_t*>(b + o_lc);
+  sc.scan_tot = reinterpret_cast<uint32_t*>(b + o_st);
+  sc.rec_lo[0] = reinterpret_cast<uint64_t*>(b + o_lo0);
+  sc.rec_lo[1] = reinterpret_cast<uint64_t*>(b + o_lo1);
+  sc.rec_meta[0] = reinterpret_cast<uint32_t*>(b + o_m0);
+  sc.rec_meta[1] = reinterpret_cast<uint32_t*>(b + o_m1);
+  uint32_t f[kFulcsTable];
+  config_fulcrums(cfg, f);
+  if (std::memcmp(f, fulcs_host, sizeof(f)) != 0) {
+    // from pageable memory: the runtime has read f when the call returns
+    TRY_HIP(hipMemcpyAsync(sc.fulcs, f, sizeof(f), hipMemcpyHostToDevice, s));
+    std::memcpy(fulcs_host, f, sizeof(f));
+  }
+  KeySource src{};
+  src.u64 = d_keys;
+  return launch_build(ctx, sc, kKindU64, src, cfg, pl0, *d_out, d_status, s, BatchInfo{}, true);
+}
+
 hipError_t phobic::dev_alloc(void** p, size_t bytes, hipStream_t s) {
   hipMemPool_t pool = nullptr;
   const hipError_t e = pool_of_current(&pool);
@@ -637,7 +722,7 @@ int batch_enqueue(phobic_batch* b, const uint64_t* d_keys, uint32_t* d_pilots, u
   bi.set_part_base = b->d_part_base.as<uint32_t>();
   bi.global_offsets = b->d_goff.as<uint32_t>();
   phobic_outputs out{d_pilots, d_offsets, nullptr, nullptr};
-  return launch_build(b->ctx, kKindU64, src, &b->cfg, b->pl, out, d_status, s, bi, false);
+  return launch_build(b->ctx, ctx_scratch(b->ctx), kKindU64, src, &b->cfg, b->pl, out, d_status, s, bi, false);
 }
 
 }  // namespace
@@ -802,6 +887,12 @@ int phobic_ctx_create(int device, phobic_ctx** out) {
   phobic_ctx* ctx = new (std::nothrow) phobic_ctx();
   if (!ctx) return PHOBIC_E_OOM;
   ctx->device = device;
+  // the device address of the allocation-free entry point's scratch
+  const int ns = phobic::noalloc_init(device);
+  if (ns != PHOBIC_OK) {
+    delete ctx;
+    return ns;
+  }
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   for (int i = 0; e == hipSuccess && i <= kStages; i++) e = hipEventCreate(&ctx->ev[i]);
   if (e != hipSuccess) {
@@ -847,6 +938,7 @@ const char* phobic_status_string(int s) {
     case PHOBIC_E_PILOT_OVERFLOW: return "no pilot below 2^31 places a bucket";
     case PHOBIC_E_BUSY: return "request rejected: queue or memory budget full";
     case PHOBIC_E_NOT_FOUND: return "unknown request";
+    case PHOBIC_E_SCRATCH_LIMIT: return "build exceeds the allocation-free entry point's fixed scratch";
     default: return "unknown status";
   }
 }

@@ -29,7 +29,9 @@ STATUS_NAMES = {
     8: "PHOBIC_E_PILOT_OVERFLOW",
     9: "PHOBIC_E_BUSY",
     10: "PHOBIC_E_NOT_FOUND",
+    11: "PHOBIC_E_SCRATCH_LIMIT",
 }
+NOALLOC_SCRATCH_BYTES = 64 << 20
 MAX_STAGES = 8
 
 
@@ -104,6 +106,7 @@ EXPORTS = [
     "phobic_build_u64_device",
     "phobic_build_bytes_device",
     "phobic_build_keys_device",
+    "phobic_build_u64_device_noalloc",
     "phobic_batch_create",
     "phobic_batch_layout",
     "phobic_batch_launch",
@@ -157,6 +160,7 @@ def load() -> ctypes.CDLL:
         "phobic_build_u64_device": (I, [P, P, U64, cfgp, op, P, P]),
         "phobic_build_bytes_device": (I, [P, P, P, U64, cfgp, op, P, P]),
         "phobic_build_keys_device": (I, [P, P, U64, cfgp, op, P, P]),
+        "phobic_build_u64_device_noalloc": (I, [P, P, U64, cfgp, op, P, P]),
         "phobic_batch_create": (I, [P, cfgp, P, U32, ctypes.POINTER(P)]),
         "phobic_batch_layout": (I, [P, P, ctypes.POINTER(U32)]),
         "phobic_batch_launch": (I, [P, P, P, P, P, P]),
@@ -323,6 +327,14 @@ class Context:
                                               _ptr(d_status), _stream(stream))
         if st:
             raise PhobicError(st, "build_u64_device")
+
+    def build_u64_device_noalloc(self, d_keys, n, cfg: Config, out: Outputs, d_status, stream=None) -> None:
+        """phobic_build_u64_device_noalloc: the same build with the library's
+        fixed per-device scratch array (no allocation in the call)."""
+        st = self.lib.phobic_build_u64_device_noalloc(self.h, _ptr(d_keys), n, ctypes.byref(cfg), ctypes.byref(out),
+                                                      _ptr(d_status), _stream(stream))
+        if st:
+            raise PhobicError(st, "build_u64_device_noalloc")
 
     def build_bytes_device(self, d_bytes, d_offs, n, cfg, out: Outputs, d_status, stream=None) -> None:
         st = self.lib.phobic_build_bytes_device(self.h, _ptr(d_bytes), _ptr(d_offs), n, ctypes.byref(cfg),
