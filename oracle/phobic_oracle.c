@@ -42,16 +42,14 @@ static const uint64_t kSecretWords[24] = {
     0xfca1477d58be162bull, 0xce31d07ad1b8f88full, 0x280416958f3acb45ull, 0x7e404bbbcafbd7afull};
 
 static uint8_t g_secret[192];
-static int g_secret_ready = 0;
 
-static const uint8_t* default_secret(void) {
-  if (!g_secret_ready) {
-    for (int i = 0; i < 24; i++)
-      for (int b = 0; b < 8; b++) g_secret[8 * i + b] = (uint8_t)(kSecretWords[i] >> (8 * b));
-    g_secret_ready = 1;
-  }
-  return g_secret;
+/* filled when the library is loaded, before any thread hashes */
+__attribute__((constructor)) static void init_secret(void) {
+  for (int i = 0; i < 24; i++)
+    for (int b = 0; b < 8; b++) g_secret[8 * i + b] = (uint8_t)(kSecretWords[i] >> (8 * b));
 }
+
+static const uint8_t* default_secret(void) { return g_secret; }
 
 static inline uint64_t rd64(const uint8_t* p) {
   uint64_t v = 0;
@@ -536,6 +534,219 @@ int orc_build_keys_ex(const orc_key* keys, uint64_t n, double lambda, uint32_t P
   free(nbk);
   free(cnt);
   free(fulcs);
+  return err;
+}
+
+/* ------------------------------------------- partitions of a key stream -- */
+
+/* splitmix64 output i+1 from state `seed` (Steele, Lea, Flood 2014). */
+static inline uint64_t splitmix_at(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static inline orc_key hash_one_u64(uint64_t v, uint64_t seed) {
+  uint8_t b[8];
+  for (int k = 0; k < 8; k++) b[k] = (uint8_t)(v >> (8 * k));
+  uint64_t lo, hi;
+  orc_xxh3_128(b, 8, seed, &lo, &hi);
+  return key_of(lo, hi);
+}
+
+/* One partition built on its own from its ps keys (any order): the same
+ * steps orc_build_keys_ex takes per partition -- bucket counts
+ * (bucket_sizes.comp:21-35), processing order size descending then index
+ * descending (sort_sum.comp:156-172), keys grouped by bucket
+ * (redistribute_keys.comp:21-53), the search (search.comp:105-175) over
+ * orc_table_size(ps) slots and, for a_ppm < 1e6, the remap of the used slots
+ * of [ps, m) onto the free slots of [0, ps).  pil[b] receives bucket b's
+ * pilot (0 when empty); remap receives m - ps entries. */
+static int build_partition(const orc_key* keys, uint32_t ps, uint32_t B, const uint32_t* fulcs, uint32_t a_ppm,
+                           uint32_t* pil, uint32_t* remap) {
+  uint32_t* cnt = (uint32_t*)calloc(B, 4);
+  uint32_t* kb = (uint32_t*)malloc((ps ? ps : 1) * 4);
+  int status = ORC_OK;
+  for (uint32_t i = 0; i < ps; i++) {
+    kb[i] = orc_bucket_of(keys[i].bucketer, fulcs);
+    cnt[kb[i]]++;
+  }
+  for (uint32_t b = 0; b < B; b++)
+    if (cnt[b] > ORC_MAX_BUCKET) status = ORC_E_BUCKET_OVERFLOW;
+  memset(pil, 0, (size_t)B * 4);
+  if (status) {
+    free(cnt);
+    free(kb);
+    return status;
+  }
+  uint32_t* ord = (uint32_t*)malloc(B * 4);
+  uint32_t* st = (uint32_t*)malloc(B * 4);
+  uint32_t* sz = (uint32_t*)malloc(B * 4);
+  uint32_t* pv = (uint32_t*)malloc(B * 4);
+  uint32_t* start_of = (uint32_t*)malloc(B * 4);
+  uint32_t m = 0, at = 0;
+  for (uint32_t size = ORC_MAX_BUCKET; size >= 1; size--)
+    for (int64_t b = (int64_t)B - 1; b >= 0; b--)
+      if (cnt[b] == size) ord[m++] = (uint32_t)b;
+  for (uint32_t j = 0; j < m; j++) {
+    st[j] = at;
+    sz[j] = cnt[ord[j]];
+    start_of[ord[j]] = at;
+    at += sz[j];
+  }
+  orc_key* grouped = (orc_key*)malloc((ps ? ps : 1) * sizeof(orc_key));
+  for (uint32_t i = 0; i < ps; i++) grouped[start_of[kb[i]]++] = keys[i];
+  const uint32_t mt = orc_table_size(ps, a_ppm);
+  const int use_remap = a_ppm < 1000000u;
+  uint64_t* taken = NULL;
+  status = search_partition(grouped, mt, m, ord, st, sz, pv, use_remap ? &taken : NULL);
+  if (!status) {
+    for (uint32_t j = 0; j < m; j++) pil[ord[j]] = pv[j];
+    if (use_remap) {
+      uint32_t hole = 0;
+      for (uint32_t q = ps; q < mt; q++) {
+        remap[q - ps] = 0;
+        if (taken[q >> 6] & (1ull << (q & 63))) {
+          while (taken[hole >> 6] & (1ull << (hole & 63))) hole++;
+          remap[q - ps] = hole++;
+        }
+      }
+    }
+  }
+  free(taken);
+  free(grouped);
+  free(start_of);
+  free(pv);
+  free(sz);
+  free(st);
+  free(ord);
+  free(kb);
+  free(cnt);
+  return status;
+}
+
+/* Per-partition key counts and largest bucket of the key stream
+ * splitmix64(kseed) outputs 1..n hashed with XXH3 seed hseed
+ * (hasher.hpp:42-48), without holding the keys: one pass with atomic
+ * counters.  max_bucket may be NULL. */
+int orc_scan_splitmix(uint64_t kseed, uint64_t n, uint64_t hseed, double lambda, uint32_t P, uint32_t* counts,
+                      uint16_t* max_bucket) {
+  if (P == 0 || !(lambda > 0) || n > 0xFFFFFFFFull || !counts) return ORC_E_INVALID;
+  uint32_t np, B;
+  orc_sizes(n, lambda, P, &np, &B);
+  uint32_t fulcs[ORC_FULCS];
+  orc_fulcrums(lambda, P, fulcs);
+  memset(counts, 0, (size_t)np * 4);
+  uint16_t* bc = max_bucket ? (uint16_t*)calloc((size_t)np * B + 1, 2) : NULL;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const orc_key k = hash_one_u64(splitmix_at(kseed, (uint64_t)i), hseed);
+    const uint32_t p = orc_partition_of(k.partitioner, np);
+    __atomic_fetch_add(&counts[p], 1u, __ATOMIC_RELAXED);
+    if (bc) {
+      const uint64_t g = (uint64_t)p * B + orc_bucket_of(k.bucketer, fulcs);
+      uint16_t c = __atomic_load_n(&bc[g], __ATOMIC_RELAXED);
+      while (c < 0xFFFFu && !__atomic_compare_exchange_n(&bc[g], &c, (uint16_t)(c + 1), 1, __ATOMIC_RELAXED,
+                                                        __ATOMIC_RELAXED)) {
+      }
+    }
+  }
+  if (bc) {
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < (int64_t)np; p++) {
+      uint16_t mx = 0;
+      for (uint32_t b = 0; b < B; b++) mx = bc[(uint64_t)p * B + b] > mx ? bc[(uint64_t)p * B + b] : mx;
+      max_bucket[p] = mx;
+    }
+    free(bc);
+  }
+  return ORC_OK;
+}
+
+/* Builds only the partitions parts[0..nparts) (distinct, any order) of the
+ * key stream of orc_scan_splitmix: pilots_sel[j*B + b] is the pilot of
+ * bucket b of partition parts[j]; counts_sel[j] its key count; for
+ * a_ppm < 1e6 remap_sel receives, for j in list order, the
+ * orc_table_size(counts_sel[j]) - counts_sel[j] remap entries of partition
+ * parts[j] (remap_sel_cap entries at most).  Each partition depends only on
+ * its own keys, so these equal the corresponding rows of a full build. */
+int orc_build_splitmix_parts(uint64_t kseed, uint64_t n, uint64_t hseed, double lambda, uint32_t P, uint32_t a_ppm,
+                             const uint32_t* parts, uint32_t nparts, uint32_t* pilots_sel, uint32_t* counts_sel,
+                             uint32_t* remap_sel, uint64_t remap_sel_cap) {
+  if (P == 0 || !(lambda > 0) || n > 0xFFFFFFFFull || a_ppm == 0 || a_ppm > 1000000u) return ORC_E_INVALID;
+  uint32_t np, B;
+  orc_sizes(n, lambda, P, &np, &B);
+  if (B == 0 || B > 65535) return ORC_E_INVALID;
+  const int use_remap = a_ppm < 1000000u;
+  if (use_remap && !remap_sel) return ORC_E_INVALID;
+  int32_t* slot = (int32_t*)malloc((size_t)(np ? np : 1) * 4);
+  for (uint32_t p = 0; p < np; p++) slot[p] = -1;
+  for (uint32_t j = 0; j < nparts; j++) {
+    if (parts[j] >= np || slot[parts[j]] >= 0) {
+      free(slot);
+      return ORC_E_INVALID;
+    }
+    slot[parts[j]] = (int32_t)j;
+  }
+  uint32_t fulcs[ORC_FULCS];
+  orc_fulcrums(lambda, P, fulcs);
+  /* pass 1: keys per listed partition; pass 2: the keys themselves */
+  uint32_t* cnt = (uint32_t*)calloc((size_t)nparts + 1, 4);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const orc_key k = hash_one_u64(splitmix_at(kseed, (uint64_t)i), hseed);
+    const int32_t j = slot[orc_partition_of(k.partitioner, np)];
+    if (j >= 0) __atomic_fetch_add(&cnt[j], 1u, __ATOMIC_RELAXED);
+  }
+  uint64_t* start = (uint64_t*)malloc(((size_t)nparts + 1) * 8);
+  uint64_t total = 0, rtotal = 0;
+  int status = ORC_OK;
+  for (uint32_t j = 0; j < nparts; j++) {
+    start[j] = total;
+    total += cnt[j];
+    counts_sel[j] = cnt[j];
+    if (cnt[j] > P + P / 2) status = merge_err(status, ORC_E_PARTITION_OVERFLOW); /* mphf_config.h:61-64 */
+    rtotal += orc_table_size(cnt[j], a_ppm) - cnt[j];
+  }
+  start[nparts] = total;
+  if (use_remap && rtotal > remap_sel_cap) status = ORC_E_INVALID;
+  if (status) {
+    free(start);
+    free(cnt);
+    free(slot);
+    return status;
+  }
+  orc_key* keys = (orc_key*)malloc((size_t)(total ? total : 1) * sizeof(orc_key));
+  uint32_t* fill = (uint32_t*)calloc((size_t)nparts + 1, 4);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; i++) {
+    const orc_key k = hash_one_u64(splitmix_at(kseed, (uint64_t)i), hseed);
+    const int32_t j = slot[orc_partition_of(k.partitioner, np)];
+    if (j >= 0) keys[start[j] + __atomic_fetch_add(&fill[j], 1u, __ATOMIC_RELAXED)] = k;
+  }
+  uint64_t* rstart = (uint64_t*)malloc(((size_t)nparts + 1) * 8);
+  rtotal = 0;
+  for (uint32_t j = 0; j < nparts; j++) {
+    rstart[j] = rtotal;
+    rtotal += orc_table_size(cnt[j], a_ppm) - cnt[j];
+  }
+  int err = ORC_OK;
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t j = 0; j < (int64_t)nparts; j++) {
+    const int s = build_partition(keys + start[j], cnt[j], B, fulcs, a_ppm, pilots_sel + (uint64_t)j * B,
+                                  use_remap ? remap_sel + rstart[j] : NULL);
+    if (s) {
+#pragma omp critical
+      err = merge_err(err, s);
+    }
+  }
+  free(rstart);
+  free(fill);
+  free(keys);
+  free(start);
+  free(cnt);
+  free(slot);
   return err;
 }
 

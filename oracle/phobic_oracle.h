@@ -80,6 +80,20 @@ void orc_query_keys_ex(const orc_key* keys, uint64_t n, const uint32_t* pilots, 
                        uint32_t np, uint32_t B, const uint32_t* fulcs, uint32_t a_ppm, const uint32_t* remap,
                        const uint32_t* remap_offsets, uint32_t* out);
 
+/* Partitions of the key stream splitmix64(kseed) outputs 1..n (hashed
+ * with XXH3 seed hseed, hasher.hpp:42-48), computed without holding all keys.
+ * orc_scan_splitmix: per-partition key counts (np) and, if max_bucket is not
+ * NULL, the largest bucket of each partition (saturating at 65535).
+ * orc_build_splitmix_parts: builds only the listed partitions, each on its
+ * own exactly as orc_build_keys_ex does: pilots_sel[j*B + b], counts_sel[j],
+ * and for a_ppm < 1e6 the concatenated remap entries of the listed
+ * partitions in list order (at most remap_sel_cap). */
+int orc_scan_splitmix(uint64_t kseed, uint64_t n, uint64_t hseed, double lambda, uint32_t partition_size,
+                      uint32_t* counts, uint16_t* max_bucket);
+int orc_build_splitmix_parts(uint64_t kseed, uint64_t n, uint64_t hseed, double lambda, uint32_t partition_size,
+                             uint32_t a_ppm, const uint32_t* parts, uint32_t nparts, uint32_t* pilots_sel,
+                             uint32_t* counts_sel, uint32_t* remap_sel, uint64_t remap_sel_cap);
+
 /* Sizes for a build: np = ceil(n/P), B = ceil(P/lambda). */
 void orc_sizes(uint64_t n, double lambda, uint32_t partition_size, uint32_t* np, uint32_t* B);
 

@@ -49,6 +49,8 @@ def load() -> ctypes.CDLL:
         "orc_compact_access": (U64, [P, U64, U64]),
         "orc_encode_mphf": (U64, [P, P, U32, U32, U32, P, U64, U64, U32, P, P]),
         "orc_rice_param": (U64, [P, U64]),
+        "orc_scan_splitmix": (ctypes.c_int, [U64, U64, U64, D, U32, P, P]),
+        "orc_build_splitmix_parts": (ctypes.c_int, [U64, U64, U64, D, U32, U32, P, U32, P, P, P, U64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -174,6 +176,38 @@ def query_keys(keys: np.ndarray, pilots: np.ndarray, offsets: np.ndarray, lam: f
     lib.orc_query_keys(keys.ctypes.data, keys.shape[0], np.ascontiguousarray(pilots).ctypes.data,
                        np.ascontiguousarray(offsets).ctypes.data, np_, B, f.ctypes.data, out.ctypes.data)
     return out
+
+
+def scan_splitmix(kseed: int, n: int, hseed: int, lam: float, P: int, max_bucket: bool = True):
+    """Per-partition key counts and largest bucket of the keys splitmix64(kseed)
+    1..n, hashed with XXH3 seed hseed, without materialising them."""
+    lib = load()
+    np_, _ = sizes(n, lam, P)
+    counts = np.zeros(max(np_, 1), np.uint32)
+    mb = np.zeros(max(np_, 1), np.uint16) if max_bucket else None
+    st = lib.orc_scan_splitmix(kseed, n, hseed, lam, P, counts.ctypes.data, mb.ctypes.data if mb is not None else None)
+    if st:
+        raise RuntimeError(f"orc_scan_splitmix: status {st}")
+    return counts[:np_], (mb[:np_] if mb is not None else None)
+
+
+def build_splitmix_parts(kseed: int, n: int, hseed: int, lam: float, P: int, alpha: float, parts):
+    """Builds only the listed partitions of the splitmix64(kseed) key stream:
+    (status, pilots[len(parts), B], counts[len(parts)], remap entries
+    concatenated in list order)."""
+    lib = load()
+    parts = np.ascontiguousarray(parts, dtype=np.uint32)
+    np_, B = sizes(n, lam, P)
+    a = alpha_ppm(alpha)
+    k = parts.size
+    pil = np.zeros((max(k, 1), B), np.uint32)
+    cnt = np.zeros(max(k, 1), np.uint32)
+    cap = int(lib.orc_remap_capacity(k * (P + P // 2), k, a)) if a < 1_000_000 else 0
+    rm = np.zeros(max(cap, 1), np.uint32)
+    st = lib.orc_build_splitmix_parts(kseed, n, hseed, lam, P, a, parts.ctypes.data if k else None, k,
+                                      pil.ctypes.data, cnt.ctypes.data, rm.ctypes.data, cap)
+    used = int(sum(int(lib.orc_table_size(int(c), a)) - int(c) for c in cnt[:k])) if (a < 1_000_000 and st == 0) else 0
+    return st, pil[:k], cnt[:k], rm[:used]
 
 
 def compact_encode(values: np.ndarray) -> tuple[np.ndarray, int]:

@@ -369,8 +369,9 @@ def test_search_modes_bit_exact(ctx, n, lam, P, alpha, kseed):
 @pytest.mark.parametrize("alpha", [1.0, 0.97])
 def test_offsets_span_several_scan_tiles(ctx, alpha):
     """20M keys at P = 2048 give 9766 partitions, three tiles of the offsets
-    scan: offsets are the exclusive scan of the partition counts and remap
-    offsets that of table_size(count) - count."""
+    scan and three routing levels: offsets are the exclusive scan of the
+    partition counts and remap offsets that of table_size(count) - count;
+    pilots and remap entries equal the oracle's."""
     n, lam, P = 20_000_000, 7.5, 2048
     keys = po.splitmix64(31, n)
     r = ctx.build_u64(keys, ph.make_config(lam, P, 0, alpha))
@@ -388,6 +389,12 @@ def test_offsets_span_several_scan_tiles(ctx, alpha):
         roff = np.zeros(np_ + 1, np.int64)
         roff[1:] = np.cumsum(extra)
         assert np.array_equal(r.remap_offsets.astype(np.int64), roff)
+        st, pil, ooff, rm, oroff = po.build_keys_alpha(hk, lam, P, alpha)
+        assert st == 0 and np.array_equal(r.remap, rm)
+    else:
+        st, pil, ooff = po.build_keys(hk, lam, P)
+        assert st == 0
+    assert np.array_equal(r.pilots, pil)
 
 
 def gen_strings_ref(seed, offs):

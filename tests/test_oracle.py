@@ -439,3 +439,36 @@ def test_rice_parameter_near_optimal_for_geometric_values(mean):
     l = po.rice_param(vals)
     assert cost[l] <= 1.01 * min(cost)
     assert po.rice_param(np.zeros(100, np.uint64)) == 0
+
+
+@pytest.mark.parametrize("n,lam,P,alpha,kseed", [
+    (300_000, 7.5, 2048, 0.99, 3),
+    (200_000, 4.0, 500, 1.0, 4),
+    (150_000, 7.0, 1024, 0.7, 5),
+])
+def test_partition_subsets_equal_full_build(n, lam, P, alpha, kseed):
+    """The streaming partition builder (used at sizes the full oracle build
+    cannot hold) equals orc_build_keys_ex row by row, for partitions listed
+    in any order, and its scan gives the full build's counts and largest
+    bucket per partition."""
+    hk = po.hash_u64(po.splitmix64(kseed, n), 0)
+    st, pil, off, rm, roff = po.build_keys_alpha(hk, lam, P, alpha)
+    assert st == 0
+    np_, B = po.sizes(n, lam, P)
+    parts = np.random.default_rng(kseed).permutation(np_)[: max(1, np_ // 3)].astype(np.uint32)
+    st2, pil2, cnt2, rm2 = po.build_splitmix_parts(kseed, n, 0, lam, P, alpha, parts)
+    assert st2 == 0
+    assert np.array_equal(pil2, pil.reshape(B, np_).T[parts])
+    assert np.array_equal(cnt2, np.diff(off.astype(np.int64))[parts])
+    if alpha < 1.0:
+        assert np.array_equal(rm2, np.concatenate([rm[roff[p]:roff[p + 1]] for p in parts]))
+    counts, maxb = po.scan_splitmix(kseed, n, 0, lam, P)
+    assert np.array_equal(counts, np.diff(off.astype(np.int64)))
+    _, _, _, bs, _ = po.build_keys(hk, lam, P, intermediates=True)
+    assert np.array_equal(maxb, bs.reshape(np_, B).max(axis=1))
+
+
+def test_partition_subset_errors():
+    # a partition listed twice or out of range is refused
+    assert po.build_splitmix_parts(1, 10_000, 0, 7.0, 2048, 1.0, [0, 0])[0] == 1
+    assert po.build_splitmix_parts(1, 10_000, 0, 7.0, 2048, 1.0, [5])[0] == 1
