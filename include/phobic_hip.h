@@ -274,11 +274,29 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
 int phobic_last_stage_times(phobic_ctx* ctx, phobic_stage_times* times);
 
 /* ---- evaluation (MPHF::operator(), include/phobicGpu/mphf.hpp:77-86) ----
- * Device-side batched query of u64 keys against (pilots, offsets) in HBM:
- * d_out[i] = position of d_keys[i] in [0, n). */
+ * Device-side batched queries against an MPHF in HBM (pilots, offsets and,
+ * for alpha < 1, the remap table, as a build wrote them): d_out[i] = the
+ * position in [0, n) of key i, computed as the reference's operator() does
+ * for the key type's initial hash (hasher.hpp:22-66).  A key that was not
+ * built and falls in a partition without keys gets offsets[p].
+ * Asynchronous on `stream`. */
+/* uint64_t keys: XXH3-128 of the 8 native bytes (hasher.hpp:42-48). */
 int phobic_eval_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t count,
                            const phobic_config* cfg, uint32_t np, const phobic_outputs* d_mphf,
                            uint32_t* d_out, void* stream);
+/* std::string keys: key i = d_bytes[d_offsets[i] .. d_offsets[i+1])
+ * (hasher.hpp:33-39). */
+int phobic_eval_bytes_device(phobic_ctx* ctx, const uint8_t* d_bytes, const uint64_t* d_offsets, uint64_t count,
+                             const phobic_config* cfg, uint32_t np, const phobic_outputs* d_mphf,
+                             uint32_t* d_out, void* stream);
+/* 16-byte keys hashed with XXH3-128: std::pair<uint64_t, uint64_t> (first,
+ * second at d_pairs[2i], d_pairs[2i+1]) or a Key under the xxhash hasher
+ * (hasher.hpp:51-66). */
+int phobic_eval_pairs_device(phobic_ctx* ctx, const uint64_t* d_pairs, uint64_t count, const phobic_config* cfg,
+                             uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out, void* stream);
+/* Pre-hashed Keys (the nohash hasher, hasher.hpp:22-26). */
+int phobic_eval_keys_device(phobic_ctx* ctx, const phobic_key* d_keys, uint64_t count, const phobic_config* cfg,
+                            uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out, void* stream);
 
 /* The validity contract of example.cpp:43-59 / benchmark.cpp:45-62 on the
  * device: d_bad (device u64) receives the number of positions in d_pos that

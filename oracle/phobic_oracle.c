@@ -412,17 +412,40 @@ int orc_build_keys(const orc_key* keys, uint64_t n, double lambda, uint32_t P, u
  * [ps_p, m_p) are remapped, in increasing order, onto the free slots of
  * [0, ps_p), also in increasing order: remap[remap_offsets[p] + q - ps_p] is
  * the slot q stands for (0 for unused q).  remap_offsets[np] = entries. */
+static int build_keys_core(const orc_key* keys, uint64_t n, uint32_t P, uint32_t B, const uint32_t* fulcs_in,
+                           uint32_t a_ppm, uint32_t* pilots, uint32_t* offsets, uint32_t* remap,
+                           uint32_t* remap_offsets, uint32_t* bucket_sizes_out, uint32_t* order_out);
+
 int orc_build_keys_ex(const orc_key* keys, uint64_t n, double lambda, uint32_t P, uint32_t a_ppm, uint32_t* pilots,
                       uint32_t* offsets, uint32_t* remap, uint32_t* remap_offsets, uint32_t* bucket_sizes_out,
                       uint32_t* order_out) {
-  if (P == 0 || !(lambda > 0) || n > 0xFFFFFFFFull || a_ppm == 0 || a_ppm > 1000000u) return ORC_E_INVALID;
+  if (P == 0 || !(lambda > 0)) return ORC_E_INVALID;
+  uint32_t fulcs[ORC_FULCS];
+  orc_fulcrums(lambda, P, fulcs);
+  return build_keys_core(keys, n, P, orc_bucket_count(lambda, P), fulcs, a_ppm, pilots, offsets, remap,
+                         remap_offsets, bucket_sizes_out, order_out);
+}
+
+/* A custom Bucketer's B and fulcrum table (mphf_config.h:34-42). */
+int orc_build_keys_table(const orc_key* keys, uint64_t n, uint32_t P, uint32_t B, const uint32_t* fulcs,
+                         uint32_t a_ppm, uint32_t* pilots, uint32_t* offsets, uint32_t* remap,
+                         uint32_t* remap_offsets) {
+  if (!fulcs) return ORC_E_INVALID;
+  for (int i = 0; i < ORC_FULCS; i++)
+    if ((uint64_t)fulcs[i] > ((uint64_t)B << 16)) return ORC_E_INVALID;
+  return build_keys_core(keys, n, P, B, fulcs, a_ppm, pilots, offsets, remap, remap_offsets, NULL, NULL);
+}
+
+static int build_keys_core(const orc_key* keys, uint64_t n, uint32_t P, uint32_t B, const uint32_t* fulcs_in,
+                           uint32_t a_ppm, uint32_t* pilots, uint32_t* offsets, uint32_t* remap,
+                           uint32_t* remap_offsets, uint32_t* bucket_sizes_out, uint32_t* order_out) {
+  if (P == 0 || n > 0xFFFFFFFFull || a_ppm == 0 || a_ppm > 1000000u) return ORC_E_INVALID;
   const int use_remap = a_ppm < 1000000u;
   if (use_remap && (!remap || !remap_offsets)) return ORC_E_INVALID;
-  uint32_t np, B;
-  orc_sizes(n, lambda, P, &np, &B);
+  const uint32_t np = (uint32_t)((n + P - 1) / P);
   if (B == 0 || B > 65535) return ORC_E_INVALID;
   uint32_t* fulcs = (uint32_t*)malloc(ORC_FULCS * 4);
-  orc_fulcrums(lambda, P, fulcs);
+  memcpy(fulcs, fulcs_in, ORC_FULCS * 4);
   uint64_t NB = (uint64_t)np * B;
   uint32_t* gb = (uint32_t*)malloc((n ? n : 1) * 4);
   uint32_t* cnt = (uint32_t*)calloc(NB + 1, 4);

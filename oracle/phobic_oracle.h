@@ -76,6 +76,11 @@ uint64_t orc_remap_capacity(uint64_t n, uint32_t np, uint32_t a_ppm);
 int orc_build_keys_ex(const orc_key* keys, uint64_t n, double lambda, uint32_t partition_size, uint32_t a_ppm,
                       uint32_t* pilots, uint32_t* offsets, uint32_t* remap, uint32_t* remap_offsets,
                       uint32_t* bucket_sizes, uint32_t* order);
+/* The same build with a custom Bucketer's B and 2048-entry fulcrum table
+ * (mphf_config.h:34-42); every entry must be at most B << 16. */
+int orc_build_keys_table(const orc_key* keys, uint64_t n, uint32_t partition_size, uint32_t B,
+                         const uint32_t* fulcs, uint32_t a_ppm, uint32_t* pilots, uint32_t* offsets,
+                         uint32_t* remap, uint32_t* remap_offsets);
 void orc_query_keys_ex(const orc_key* keys, uint64_t n, const uint32_t* pilots, const uint32_t* offsets,
                        uint32_t np, uint32_t B, const uint32_t* fulcs, uint32_t a_ppm, const uint32_t* remap,
                        const uint32_t* remap_offsets, uint32_t* out);

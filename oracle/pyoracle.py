@@ -49,6 +49,7 @@ def load() -> ctypes.CDLL:
         "orc_compact_access": (U64, [P, U64, U64]),
         "orc_encode_mphf": (U64, [P, P, U32, U32, U32, P, U64, U64, U32, P, P]),
         "orc_rice_param": (U64, [P, U64]),
+        "orc_build_keys_table": (ctypes.c_int, [P, U64, U32, U32, P, U32, P, P, P, P]),
         "orc_scan_splitmix": (ctypes.c_int, [U64, U64, U64, D, U32, P, P]),
         "orc_build_splitmix_parts": (ctypes.c_int, [U64, U64, U64, D, U32, U32, P, U32, P, P, P, U64]),
     }
@@ -148,6 +149,39 @@ def build_keys_alpha(keys: np.ndarray, lam: float, P: int, alpha: float):
     kp = keys.ctypes.data if n else np.zeros(4, np.uint32).ctypes.data
     st = lib.orc_build_keys_ex(kp, n, lam, P, a, pilots.ctypes.data, offsets.ctypes.data, remap.ctypes.data,
                                roff.ctypes.data, None, None)
+    used = int(roff[-1]) if a < 1_000_000 else 0
+    return st, pilots, offsets, remap[:used], roff
+
+
+def bucketer_fulcrums(kind: str, lam: float, P: int) -> tuple[int, np.ndarray]:
+    """(B, table) of MPHFconfig(Bucketer*, lambda, P) for the Uniform or Skew
+    bucketer: B = round(P / lambda), f[i] = (u32)(y(i/2047) * (B << 16)),
+    f[0] = 0, f[2047] = B << 16 (mphf_config.h:34-55, bucketers.h:18-50)."""
+    B = int(np.floor(P / lam + 0.5))  # C round(): halves away from zero
+    y = {"uniform": lambda x: x,
+         "skew": lambda x: x / 0.6 * 0.3 if x < 0.6 else 0.3 + (x - 0.6) / 0.4 * 0.7}[kind]
+    f = np.zeros(2048, np.uint32)
+    for i in range(1, 2047):
+        f[i] = int(y(i / 2047.0) * float(B << 16))
+    f[2047] = B << 16
+    return B, f
+
+
+def build_keys_table(keys: np.ndarray, P: int, B: int, fulcs: np.ndarray, alpha: float = 1.0):
+    """Build with a custom fulcrum table: (status, pilots, offsets, remap, remap_offsets)."""
+    lib = load()
+    keys = np.ascontiguousarray(keys, dtype=np.uint32).reshape(-1, 4)
+    n = keys.shape[0]
+    np_ = (n + P - 1) // P
+    a = alpha_ppm(alpha)
+    pilots = np.zeros(np_ * B, dtype=np.uint32)
+    offsets = np.zeros(np_ + 1, dtype=np.uint32)
+    remap = np.zeros(max(1, lib.orc_remap_capacity(n, np_, a)), dtype=np.uint32)
+    roff = np.zeros(np_ + 1, dtype=np.uint32)
+    f = np.ascontiguousarray(fulcs, dtype=np.uint32)
+    kp = keys.ctypes.data if n else np.zeros(4, np.uint32).ctypes.data
+    st = lib.orc_build_keys_table(kp, n, P, B, f.ctypes.data, a, pilots.ctypes.data, offsets.ctypes.data,
+                                  remap.ctypes.data, roff.ctypes.data)
     used = int(roff[-1]) if a < 1_000_000 else 0
     return st, pilots, offsets, remap[:used], roff
 

@@ -137,26 +137,50 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(const uint32_t* __r
 }
 
 // ------------------------------------------------------------- evaluate ---
-// MPHF::operator() (mphf.hpp:32-40,77-86) over a device batch of u64 keys.
+// MPHF::operator() (mphf.hpp:77-86, hashPos mphf.hpp:32-40) over a device
+// batch of keys: the initial hash of the key type (hasher.hpp:22-66), its
+// partition and bucket, then offsets[p] + ((h >> 1) + V) mod m.  A key in a
+// partition without keys (a key outside the build) gets offsets[p].
 
-__global__ __launch_bounds__(256) void k_eval_u64(EvalArgs a) {
+template <int KIND>
+__device__ __forceinline__ U128 eval_hash(const KeySource& src, uint64_t i, uint64_t seed) {
+  if constexpr (KIND == kKindU64) {
+    return xxh3_128_u64(src.u64[i], seed);
+  } else if constexpr (KIND == kKindBytes) {
+    const uint64_t a = src.offs[i], b = src.offs[i + 1];
+    return xxh3_128_bytes(src.bytes + a, b - a, seed);
+  } else if constexpr (KIND == kKindPair) {
+    return xxh3_128_bytes(reinterpret_cast<const uint8_t*>(src.keys + i), 16, seed);
+  } else {
+    const uint4 k = src.keys[i];
+    return U128{(uint64_t)k.x | ((uint64_t)k.y << 32), (uint64_t)k.z | ((uint64_t)k.w << 32)};
+  }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_eval(EvalArgs a) {
   __shared__ uint32_t fl[kFulcs];
   for (uint32_t i = threadIdx.x; i < kFulcs; i += blockDim.x) fl[i] = a.fulcs[i];
   __syncthreads();
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const bool remap = a.a_ppm < 1000000u;
   for (; i < a.count; i += stride) {
-    const U128 h = xxh3_128_u64(a.keys[i], a.seed);
+    const U128 h = eval_hash<KIND>(a.src, i, a.seed);
     const uint32_t p = partition_of((uint32_t)h.lo, a.np);
     const uint32_t b = bucket_of((uint32_t)(h.lo >> 32), fl);
-    const uint32_t V = a.pilots[p + (uint64_t)b * a.np];
     const uint32_t off = a.offsets[p];
     const uint32_t ps = a.offsets[p + 1] - off;
+    if (ps == 0) {
+      a.out[i] = off;
+      continue;
+    }
+    const uint32_t V = a.pilots[p + (uint64_t)b * a.np];
     const uint32_t tm = table_size(ps, a.a_ppm);
     const uint32_t hp = V / tm;
     const uint32_t hv = mix32((uint32_t)h.hi, mix32((uint32_t)(h.hi >> 32), hp)) >> 1;
     uint32_t pos = (hv + V) % tm;
-    if (pos >= ps) pos = a.remap[a.remap_offsets[p] + pos - ps];
+    if (remap && pos >= ps) pos = a.remap[a.remap_offsets[p] + pos - ps];
     a.out[i] = off + pos;
   }
 }
@@ -272,9 +296,15 @@ hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_eval_u64(const EvalArgs& a, hipStream_t s) {
+hipError_t launch_eval(int kind, const EvalArgs& a, hipStream_t s) {
   if (a.count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval_u64, dim3(stream_grid(a.count, 256, 256 * 32)), dim3(256), 0, s, a);
+  const dim3 grid(stream_grid(a.count, 256, 256 * 32)), block(256);
+  switch (kind) {
+    case kKindU64: hipLaunchKernelGGL(k_eval<kKindU64>, grid, block, 0, s, a); break;
+    case kKindBytes: hipLaunchKernelGGL(k_eval<kKindBytes>, grid, block, 0, s, a); break;
+    case kKindPair: hipLaunchKernelGGL(k_eval<kKindPair>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_eval<kKindKey>, grid, block, 0, s, a); break;
+  }
   return hipGetLastError();
 }
 

@@ -182,6 +182,12 @@ int validate(const phobic_config* cfg, uint64_t n, Plan* pl) {
   if ((cfg->buckets != 0) != (cfg->fulcrums != nullptr)) return PHOBIC_E_INVALID_CONFIG;
   const double Bd = cfg->buckets ? (double)cfg->buckets : std::ceil((double)cfg->partition_size / cfg->lambda);
   if (!(Bd >= 1.0) || Bd > 65535.0) return PHOBIC_E_INVALID_CONFIG;
+  // a custom table maps a key to (v1 + v2) >> 16 with v1 + v2 below the
+  // table's largest entry (bucket_of): entries up to B << 16 keep every
+  // bucket index below B
+  if (cfg->fulcrums)
+    for (uint32_t i = 0; i < kFulcsTable; i++)
+      if ((uint64_t)cfg->fulcrums[i] > ((uint64_t)Bd << 16)) return PHOBIC_E_INVALID_CONFIG;
   pl->n = n;
   pl->B = (uint32_t)Bd;
   pl->np = (uint32_t)((n + cfg->partition_size - 1) / cfg->partition_size);
@@ -1146,17 +1152,21 @@ int phobic_last_stage_times(phobic_ctx* ctx, phobic_stage_times* times) {
   return fill_times(ctx, times);
 }
 
-int phobic_eval_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t count, const phobic_config* cfg,
-                           uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int eval_device(phobic_ctx* ctx, int kind, const KeySource& src, uint64_t count, const phobic_config* cfg,
+                uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out, void* stream) {
   Plan pl;
   TRY(validate(cfg, 0, &pl));
-  if (!ctx || !d_mphf || (count && (!d_keys || !d_mphf->pilots || !d_mphf->offsets || !d_out)) || np == 0)
+  if (!ctx || !d_mphf || np == 0 || (count && (!d_mphf->pilots || !d_mphf->offsets || !d_out)))
     return PHOBIC_E_INVALID_CONFIG;
   if (pl.a_ppm < 1000000u && (!d_mphf->remap || !d_mphf->remap_offsets)) return PHOBIC_E_INVALID_CONFIG;
   TRY(set_device(ctx));
   TRY(upload_fulcrums(ctx, cfg, (hipStream_t)stream));
   EvalArgs ea{};
-  ea.keys = d_keys;
+  ea.src = src;
   ea.count = count;
   ea.seed = cfg->seed;
   ea.np = np;
@@ -1167,8 +1177,46 @@ int phobic_eval_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t cou
   ea.remap = d_mphf->remap;
   ea.remap_offsets = d_mphf->remap_offsets;
   ea.out = d_out;
-  TRY_HIP(launch_eval_u64(ea, (hipStream_t)stream));
+  TRY_HIP(launch_eval(kind, ea, (hipStream_t)stream));
   return PHOBIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int phobic_eval_u64_device(phobic_ctx* ctx, const uint64_t* d_keys, uint64_t count, const phobic_config* cfg,
+                           uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out, void* stream) {
+  if (count && !d_keys) return PHOBIC_E_INVALID_CONFIG;
+  KeySource src{};
+  src.u64 = d_keys;
+  return eval_device(ctx, kKindU64, src, count, cfg, np, d_mphf, d_out, stream);
+}
+
+int phobic_eval_bytes_device(phobic_ctx* ctx, const uint8_t* d_bytes, const uint64_t* d_offsets, uint64_t count,
+                             const phobic_config* cfg, uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out,
+                             void* stream) {
+  if (count && !d_offsets) return PHOBIC_E_INVALID_CONFIG;
+  KeySource src{};
+  src.bytes = d_bytes;
+  src.offs = d_offsets;
+  return eval_device(ctx, kKindBytes, src, count, cfg, np, d_mphf, d_out, stream);
+}
+
+int phobic_eval_pairs_device(phobic_ctx* ctx, const uint64_t* d_pairs, uint64_t count, const phobic_config* cfg,
+                             uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out, void* stream) {
+  if (count && !d_pairs) return PHOBIC_E_INVALID_CONFIG;
+  KeySource src{};
+  src.keys = reinterpret_cast<const uint4*>(d_pairs);
+  return eval_device(ctx, kKindPair, src, count, cfg, np, d_mphf, d_out, stream);
+}
+
+int phobic_eval_keys_device(phobic_ctx* ctx, const phobic_key* d_keys, uint64_t count, const phobic_config* cfg,
+                            uint32_t np, const phobic_outputs* d_mphf, uint32_t* d_out, void* stream) {
+  if (count && !d_keys) return PHOBIC_E_INVALID_CONFIG;
+  KeySource src{};
+  src.keys = reinterpret_cast<const uint4*>(d_keys);
+  return eval_device(ctx, kKindKey, src, count, cfg, np, d_mphf, d_out, stream);
 }
 
 int phobic_check_bijection_device(phobic_ctx* ctx, const uint32_t* d_pos, uint64_t count, uint64_t n,

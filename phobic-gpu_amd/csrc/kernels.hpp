@@ -8,7 +8,10 @@ namespace phobic {
 
 constexpr uint32_t kFulcsTable = 2048;  // shaders/constants.glsl:1 FULCS_INTER
 
-enum : int { kKindU64 = 0, kKindBytes = 1, kKindKey = 2 };
+// key kinds: u64 keys, byte strings, pre-hashed Keys (the reference's
+// nohash), and (evaluation only) 16-byte keys hashed with XXH3:
+// std::pair<u64, u64> and an xxhash-hashed Key (hasher.hpp:51-66)
+enum : int { kKindU64 = 0, kKindBytes = 1, kKindKey = 2, kKindPair = 3 };
 
 // Error bits raised by kernels (bit index = phobic_status code).
 enum : uint32_t {
@@ -96,7 +99,7 @@ enum : uint32_t {
 };
 
 struct EvalArgs {
-  const uint64_t* keys;
+  KeySource src;  // u64 / bytes + offs / keys (pairs: keys, hashed)
   uint64_t count;
   uint64_t seed;
   uint32_t np;
@@ -123,7 +126,7 @@ hipError_t launch_partition_offsets(const uint32_t* counts, uint32_t np, uint32_
 // waves = partitions (wave64 each) per workgroup: 1, 2, 4 or 8
 size_t search_lds_bytes(uint32_t tcap, uint32_t B, uint32_t waves);
 hipError_t launch_partition_search(const SearchArgs& a, uint32_t waves, hipStream_t s);
-hipError_t launch_eval_u64(const EvalArgs& a, hipStream_t s);
+hipError_t launch_eval(int kind, const EvalArgs& a, hipStream_t s);
 // per-set exclusive offsets from the global ones: set s's np_s + 1 entries
 // start at out + set_part_base[s] + s
 hipError_t launch_batch_offsets(const uint32_t* global_offsets, const uint32_t* set_part_base, uint32_t sets,

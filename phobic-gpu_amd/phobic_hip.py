@@ -126,6 +126,9 @@ EXPORTS = [
     "phobic_autotune",
     "phobic_last_stage_times",
     "phobic_eval_u64_device",
+    "phobic_eval_bytes_device",
+    "phobic_eval_pairs_device",
+    "phobic_eval_keys_device",
     "phobic_check_bijection_device",
     "phobic_gen_splitmix64_device",
     "phobic_gen_strings_device",
@@ -180,6 +183,9 @@ def load() -> ctypes.CDLL:
         "phobic_autotune": (I, [P, cfgp, U64, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]),
         "phobic_last_stage_times": (I, [P, tp]),
         "phobic_eval_u64_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
+        "phobic_eval_bytes_device": (I, [P, P, P, U64, cfgp, U32, op, P, P]),
+        "phobic_eval_pairs_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
+        "phobic_eval_keys_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
         "phobic_check_bijection_device": (I, [P, P, U64, U64, P, P, P]),
         "phobic_gen_splitmix64_device": (I, [U64, U64, P, P]),
         "phobic_gen_strings_device": (I, [U64, U64, P, P, P]),
@@ -195,6 +201,18 @@ def load() -> ctypes.CDLL:
 def make_config(lam: float = 8.0, partition_size: int = 2048, seed: int = 0, alpha: float = 1.0) -> Config:
     """MPHFconfig(averageBucketSize, partitionSize) defaults (mphf_config.h:26)."""
     return Config(float(lam), int(partition_size), 0, int(seed), None, float(alpha))
+
+
+def make_config_table(B: int, fulcrums: np.ndarray, lam: float = 8.0, partition_size: int = 2048, seed: int = 0,
+                      alpha: float = 1.0) -> Config:
+    """MPHFconfig(Bucketer*, ...) (mphf_config.h:34-42): a custom bucketer's
+    B and 2048-entry fulcrum table.  The Config keeps the table alive."""
+    f = np.ascontiguousarray(fulcrums, dtype=np.uint32)
+    if f.size != 2048:
+        raise ValueError("a fulcrum table has 2048 entries")
+    cfg = Config(float(lam), int(partition_size), int(B), int(seed), f.ctypes.data, float(alpha))
+    cfg._keep = f
+    return cfg
 
 
 def remap_capacity(cfg: Config, n: int) -> int:
@@ -379,6 +397,24 @@ class Context:
                                              ctypes.byref(mphf), _ptr(d_out), _stream(stream))
         if st:
             raise PhobicError(st, "eval_u64_device")
+
+    def eval_bytes_device(self, d_bytes, d_offs, count, cfg, np_, mphf: Outputs, d_out, stream=None) -> None:
+        st = self.lib.phobic_eval_bytes_device(self.h, _ptr(d_bytes), _ptr(d_offs), count, ctypes.byref(cfg), np_,
+                                               ctypes.byref(mphf), _ptr(d_out), _stream(stream))
+        if st:
+            raise PhobicError(st, "eval_bytes_device")
+
+    def eval_pairs_device(self, d_pairs, count, cfg, np_, mphf: Outputs, d_out, stream=None) -> None:
+        st = self.lib.phobic_eval_pairs_device(self.h, _ptr(d_pairs), count, ctypes.byref(cfg), np_,
+                                               ctypes.byref(mphf), _ptr(d_out), _stream(stream))
+        if st:
+            raise PhobicError(st, "eval_pairs_device")
+
+    def eval_keys_device(self, d_keys, count, cfg, np_, mphf: Outputs, d_out, stream=None) -> None:
+        st = self.lib.phobic_eval_keys_device(self.h, _ptr(d_keys), count, ctypes.byref(cfg), np_,
+                                              ctypes.byref(mphf), _ptr(d_out), _stream(stream))
+        if st:
+            raise PhobicError(st, "eval_keys_device")
 
     def check_bijection_device(self, d_pos, count, n, d_bitmap, d_bad, stream=None) -> None:
         st = self.lib.phobic_check_bijection_device(

@@ -77,6 +77,13 @@ def make_fulcrums() -> None:
         B, maxsize = map(int, lines[0].split())
         out[f"f_{lam}_{P}"] = np.array(list(map(int, lines[1].split())), dtype=np.uint32)
         out[f"b_{lam}_{P}"] = np.array([B, maxsize], dtype=np.uint32)
+    # MPHFconfig(Bucketer*, lambda, P) with the uniform and skew bucketers
+    for kind in ("uniform", "skew"):
+        for lam, P in ((7.0, 2048), (4.5, 1000)):
+            lines = subprocess.run([exe, repr(lam), str(P), kind], check=True, capture_output=True,
+                                   text=True).stdout.split("\n")
+            out[f"{kind}_{lam}_{P}"] = np.array(list(map(int, lines[1].split())), dtype=np.uint32)
+            out[f"{kind}b_{lam}_{P}"] = np.array([int(lines[0].split()[0])], dtype=np.uint32)
     np.savez_compressed(os.path.join(GOLD, "fulcrums.npz"), **out)
 
 
@@ -92,7 +99,11 @@ def make_c1() -> None:
 
 
 if __name__ == "__main__":
-    make_xxh3()
-    make_fulcrums()
-    make_c1()
+    which = sys.argv[1:] or ["xxh3", "fulcrums", "c1"]
+    if "xxh3" in which:
+        make_xxh3()
+    if "fulcrums" in which:
+        make_fulcrums()
+    if "c1" in which:
+        make_c1()
     print("golden fixtures written to", GOLD)

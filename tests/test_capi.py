@@ -57,3 +57,17 @@ def test_context_without_device_reports_no_device():
     with pytest.raises(ph.PhobicError) as e:
         ph.Context(0)
     assert e.value.status == 4
+
+
+def test_custom_fulcrum_table_bounds():
+    """A custom table is accepted with every entry at most B << 16 (then every
+    bucket index stays below B) and refused otherwise."""
+    import pyoracle as po
+
+    B, f = po.bucketer_fulcrums("uniform", 7.0, 2048)
+    cfg = ph.make_config_table(B, f, 7.0, 2048)
+    assert ph.sizes(cfg, 1_000_000) == (489, B)
+    bad = f.copy()
+    bad[2047] = (B << 16) + 1
+    with pytest.raises(ph.PhobicError):
+        ph.sizes(ph.make_config_table(B, bad, 7.0, 2048), 1_000_000)

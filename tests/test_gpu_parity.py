@@ -451,3 +451,23 @@ def test_device_string_generator_and_build(ctx):
     st, opil, ooff = po.build_keys(po.hash_bytes(data, offs, 0), 6.0, 2048)
     assert st == 0
     assert np.array_equal(r.offsets, ooff) and np.array_equal(r.pilots, opil)
+
+
+@pytest.mark.parametrize("kind,lam,P,alpha", [
+    ("uniform", 7.0, 2048, 1.0),
+    ("skew", 4.5, 1000, 1.0),
+    ("skew", 7.0, 2048, 0.99),
+])
+def test_custom_bucketer_table_bit_exact(ctx, kind, lam, P, alpha):
+    """MPHFconfig(Bucketer*, lambda, P) (mphf_config.h:34-42): a custom B and
+    fulcrum table give the oracle's pilots, offsets and remap table."""
+    n = 400_000
+    B, f = po.bucketer_fulcrums(kind, lam, P)
+    keys = po.splitmix64(17, n)
+    r = ctx.build_u64(keys, ph.make_config_table(B, f, lam, P, 0, alpha))
+    assert r.B == B
+    st, pil, off, rm, roff = po.build_keys_table(po.hash_u64(keys, 0), P, B, f, alpha)
+    assert st == 0
+    assert np.array_equal(r.offsets, off) and np.array_equal(r.pilots, pil)
+    if alpha < 1.0:
+        assert np.array_equal(r.remap_offsets, roff) and np.array_equal(r.remap, rm)

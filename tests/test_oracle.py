@@ -472,3 +472,34 @@ def test_partition_subset_errors():
     # a partition listed twice or out of range is refused
     assert po.build_splitmix_parts(1, 10_000, 0, 7.0, 2048, 1.0, [0, 0])[0] == 1
     assert po.build_splitmix_parts(1, 10_000, 0, 7.0, 2048, 1.0, [5])[0] == 1
+
+
+def test_custom_bucketer_tables_match_reference():
+    """MPHFconfig(Bucketer*, lambda, P) with the reference's UniformBucketer and
+    SkewBucketer (tables printed by the reference-built ref_config, committed
+    in fulcrums.npz): B = round(P / lambda) and the same 2048 entries."""
+    g = np.load(os.path.join(GOLD, "fulcrums.npz"))
+    for kind in ("uniform", "skew"):
+        for lam, P in ((7.0, 2048), (4.5, 1000)):
+            B, f = po.bucketer_fulcrums(kind, lam, P)
+            assert B == int(g[f"{kind}b_{lam}_{P}"][0])
+            assert np.array_equal(f, g[f"{kind}_{lam}_{P}"]), (kind, lam, P)
+
+
+def test_custom_table_build_is_a_bijection():
+    """A build with a skew table differs from the OptBucketer build and is a
+    bijection (the oracle's query uses the same table)."""
+    n, P = 200_000, 2048
+    B, f = po.bucketer_fulcrums("skew", 7.0, P)
+    hk = po.hash_u64(po.splitmix64(13, n), 0)
+    st, pil, off, _, _ = po.build_keys_table(hk, P, B, f)
+    assert st == 0
+    lib = po.load()
+    q = np.zeros(n, np.uint32)
+    lib.orc_query_keys(hk.ctypes.data, n, pil.ctypes.data, off.ctypes.data, off.size - 1, B, f.ctypes.data,
+                       q.ctypes.data)
+    assert np.unique(q).size == n and int(q.max()) < n
+    # an entry above B << 16 would map keys past the last bucket: refused
+    bad = f.copy()
+    bad[1000] = (B << 16) + 1
+    assert po.build_keys_table(hk, P, B, bad)[0] == 1
