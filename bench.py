@@ -299,7 +299,8 @@ def bench_serve(args) -> int:
 def apply_tuning_flags(ctx, args) -> None:
     """The --search-mode / --search-waves / ... overrides, on top of ctx's tuning."""
     overrides = {"search_mode": args.search_mode, "search_waves": args.search_waves, "route_per": args.route_per,
-                 "max_fanout": args.max_fanout, "split_per": args.split_per, "first_fanout": args.first_fanout}
+                 "max_fanout": args.max_fanout, "split_per": args.split_per, "first_fanout": args.first_fanout,
+                 "route_threads": args.route_threads}
     if any(v is not None for v in overrides.values()):
         t = ctx.tuning()
         t.update({k: v for k, v in overrides.items() if v is not None})
@@ -388,24 +389,30 @@ def bench_strings(args) -> int:
 
 
 def bench_copy(args) -> int:
-    """A device-to-device copy of 1 GiB: the known-size reference for scaling
-    rocprofv3 FETCH_SIZE / WRITE_SIZE readings to HBM bytes."""
+    """Copies of 1 GiB device to device, by the runtime (hipMemcpyAsync) and
+    by the library's hand-written streaming kernel (phobic_copy_device): the
+    known-size references for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE
+    readings to HBM bytes."""
     nbytes = 1 << 30
     src, dst = hip_rt.DeviceBuffer(nbytes), hip_rt.DeviceBuffer(nbytes)
     src.zero()
     hip_rt.synchronize()
     rt = hip_rt.rt()
     e0, e1 = hip_rt.Event(), hip_rt.Event()
-    ms = []
-    for _ in range(max(1, args.steps)):
-        e0.record()
-        hip_rt.check(rt.hipMemcpyAsync(dst.ptr, src.ptr, nbytes, hip_rt.D2D, None), "hipMemcpyAsync")
-        e1.record()
-        e1.synchronize()
-        ms.append(e0.elapsed_ms(e1))
-    best = min(ms)
-    print(json.dumps({"mode": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "best_ms": round(best, 4),
-                      "gbs_read_plus_write": round(2 * nbytes / best / 1e6, 1)}), flush=True)
+    res = {"mode": "copy", "bytes_read": nbytes, "bytes_written": nbytes}
+    for name, fn in (("runtime", lambda: hip_rt.check(rt.hipMemcpyAsync(dst.ptr, src.ptr, nbytes, hip_rt.D2D, None),
+                                                      "hipMemcpyAsync")),
+                     ("kernel", lambda: ph.copy_device(dst, src, nbytes))):
+        ms = []
+        for _ in range(max(1, args.steps)):
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ms.append(e0.elapsed_ms(e1))
+        best = min(ms)
+        res[name] = {"best_ms": round(best, 4), "gbs_read_plus_write": round(2 * nbytes / best / 1e6, 1)}
+    print(json.dumps(res), flush=True)
     return 0
 
 
@@ -430,10 +437,11 @@ def main() -> int:
                     help="sample keys for the launch-shape autotuner (0 = default shapes)")
     ap.add_argument("--search-mode", type=int, default=None, help="pilot-search variant mask (0..3)")
     ap.add_argument("--search-waves", type=int, default=None, help="partitions per search workgroup")
-    ap.add_argument("--route-per", type=int, default=None, help="records per thread of a level-1 routing tile (8, 12, 16 or 20)")
-    ap.add_argument("--max-fanout", type=int, default=None, help="bins per partitioning level (16..128)")
+    ap.add_argument("--route-per", type=int, default=None, help="records per thread of a level-1 routing tile")
+    ap.add_argument("--max-fanout", type=int, default=None, help="bins per partitioning level (16..1024)")
     ap.add_argument("--split-per", type=int, default=None, help="records per thread of a split tile (0 = route-per)")
     ap.add_argument("--first-fanout", type=int, default=None, help="at most this many level-1 bins (0 = max-fanout)")
+    ap.add_argument("--route-threads", type=int, default=None, help="threads of a routing workgroup (512 or 1024)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build path runs on a single device; --gpus {args.gpus} is not "

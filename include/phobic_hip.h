@@ -236,19 +236,22 @@ int phobic_server_destroy(phobic_server* server);
 /* ---- launch shapes and the autotuner ----
  * search_waves: partitions (one wave64 each) per search workgroup (1,2,4,8);
  * route_per: records per thread of a level-1 (hash and route) tile of u64
- *   or pre-hashed keys (8, 12, 16 or 20); byte strings always use 8;
- * max_fanout: bins per partitioning level (16..128);
+ *   or pre-hashed keys; byte strings always use tiles of 512 x 8;
+ * max_fanout: bins per partitioning level (16..1024);
  * search_mode: pilot-search variants, a mask of PHOBIC_SEARCH_* (0 = all
  *   of the fast paths);
- * split_per: records per thread of a tile of levels 2..L (8, 12 or 16; 0 =
- *   route_per, at most 16);
+ * split_per: records per thread of a tile of levels 2..L (0 = route_per);
  * first_fanout: at most this many level-1 bins (0 = max_fanout); the later
- *   levels then take the larger share of the fan-out.
- * Defaults {1, 20, 128, 0, 12, 48}.
- * phobic_autotune times the partitioning shapes {8,12,16,20} x {8,12,16} x
- * {0,16,32,48,64} x {64,128} (route_per x split_per x first_fanout x
- * max_fanout, on the partitioning stages' device time) and then the search
- * shapes {1,2,4,8} x {0,1,2,3} (on the search stage's time) on
+ *   levels then take the larger share of the fan-out;
+ * route_threads: threads of a routing workgroup, 512 (0) or 1024; with 512
+ *   route_per and split_per are 8, 12, 16, 20 or 24, with 1024 they are 4,
+ *   6, 8, 10 or 12.  A level whose tile would not fit the workgroup's LDS
+ *   uses the largest smaller per that does.
+ * Defaults {1, 12, 1024, 0, 12, 0, 1024}: tiles of 12,288 records, and two
+ * levels of about sqrt(np) bins for a billion keys.
+ * phobic_autotune times eight tile shapes x first_fanout {0, 48, 512} x
+ * max_fanout {128, 1024} (on the partitioning stages' device time) and then
+ * the search shapes {1,2,4,8} x {0,1,2,3} (on the search stage's time) on
  * `sample_keys` device-generated keys built with `cfg`, keeps the fastest
  * on the context and returns it.  Results do not depend on the tuning.  The
  * sample should span a few 1e4 partitions (tens of millions of keys): on
@@ -264,6 +267,7 @@ typedef struct {
   uint32_t search_mode;
   uint32_t split_per;
   uint32_t first_fanout;
+  uint32_t route_threads;
 } phobic_tuning;
 int phobic_get_tuning(phobic_ctx* ctx, phobic_tuning* tuning);
 int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* tuning);
@@ -304,6 +308,12 @@ int phobic_eval_keys_device(phobic_ctx* ctx, const phobic_key* d_keys, uint64_t 
  * ceil(n/32) u32 words.  Asynchronous on `stream`. */
 int phobic_check_bijection_device(phobic_ctx* ctx, const uint32_t* d_pos, uint64_t count, uint64_t n,
                                   uint32_t* d_bitmap, unsigned long long* d_bad, void* stream);
+
+/* Device-to-device copy by a plain streaming kernel (16-byte words; both
+ * addresses 16-byte aligned, bytes a multiple of 16), asynchronous on
+ * `stream`.  No counterpart in the reference: it is the known-size kernel
+ * against which HBM traffic counters (FETCH_SIZE, WRITE_SIZE) are scaled. */
+int phobic_copy_device(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
 
 /* ---- synthetic inputs (device-side generators used by benchmarks) ----
  * d_keys[i] = splitmix64 output number i+1 from state `seed`

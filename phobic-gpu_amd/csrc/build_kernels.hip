@@ -222,6 +222,14 @@ __global__ __launch_bounds__(256) void k_batch_offsets(const uint32_t* __restric
   out[t] = goff[pb[lo] + j] - goff[pb[lo]];
 }
 
+// Plain streaming copy of 16-byte words (the known-size reference for the
+// HBM traffic counters): every word read once and written once.
+__global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 // splitmix64 (Steele, Lea, Flood 2014): output i uses state seed + (i+1)*gamma.
 __global__ __launch_bounds__(256) void k_splitmix64(uint64_t seed, uint64_t n, uint64_t* out) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -321,6 +329,13 @@ hipError_t launch_check_bijection(const uint32_t* pos, uint64_t count, uint64_t 
   if (count == 0) return hipSuccess;
   hipLaunchKernelGGL(k_check_bijection, dim3(stream_grid(count, 256, 256 * 32)), dim3(256), 0, s, pos, count, n,
                      bitmap, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy16(void* dst, const void* src, uint64_t words, hipStream_t s) {
+  if (words == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy16, dim3(stream_grid(words, 256, 256 * 64)), dim3(256), 0, s, static_cast<uint4*>(dst),
+                     static_cast<const uint4*>(src), words);
   return hipGetLastError();
 }
 

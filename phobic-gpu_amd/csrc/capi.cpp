@@ -106,7 +106,7 @@ struct phobic_ctx {
   hipEvent_t pin_ev[2] = {};
   hipEvent_t ev[kStages + 1] = {};
   bool timed = false;
-  phobic_tuning tuning{1, 20, kMaxFanout, 0, 12, 48};
+  phobic_tuning tuning{1, 12, kMaxFanout, 0, 12, 0, 1024};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
 };
@@ -348,8 +348,12 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
                  const BatchInfo& batch, bool timed) {
   const Plan pl = tuned(ctx, pl0);
   const uint32_t per = ctx->tuning.route_per;
-  // split tiles come in 8, 12 and 16 records per thread
-  const uint32_t split_per = ctx->tuning.split_per ? ctx->tuning.split_per : std::min(per, 16u);
+  const uint32_t split_per = ctx->tuning.split_per ? ctx->tuning.split_per : per;
+  // plain u64 builds carry the raw key through every level but the last;
+  // the others carry hashed records; the last level writes what the search
+  // reads (lower64 + bucket16)
+  const bool raw = kind == kKindU64 && !batch.sets;
+  auto fmt = [&](uint32_t l) { return l == pl.L ? kFmtFinal : raw ? kFmtRaw : kFmtRec; };
   uint32_t* d_pilots = d_out.pilots;
   const bool remap = pl.a_ppm < 1000000u;
   const uint32_t np = pl.np, L = pl.L, f = pl.fanout;
@@ -381,7 +385,8 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
   ha.sets = batch.sets;
   ha.set_key_off = batch.set_key_off;
   ha.set_part_base = batch.set_part_base;
-  TRY_HIP(launch_hash_route(kind, ha, per, s));
+  const uint32_t threads = ctx->tuning.route_threads ? ctx->tuning.route_threads : 512;
+  TRY_HIP(launch_hash_route(kind, fmt(1), ha, threads, per, s));
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[1], s));
 
   for (uint32_t l = 2; l <= L; l++) {
@@ -391,15 +396,21 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
     sp.in_meta = sc.rec_meta[out_buf(l - 1)];
     sp.parent_count = counts(l - 1);
     sp.parent_cap = pl.G[l - 1] * pl.cap;
-    sp.slices = (uint32_t)((sp.parent_cap + route_tile_records(split_per) - 1) / route_tile_records(split_per));
+    const uint32_t sper = split_tile_per(threads, split_per, fmt(l - 1), fmt(l), f);
+    const uint64_t tile = (uint64_t)threads * sper;
+    sp.slices = (uint32_t)((sp.parent_cap + tile - 1) / tile);
     sp.fanout = f;
+    sp.G_parent = (uint32_t)pl.G[l - 1];
     sp.G_child = (uint32_t)pl.G[l];
     sp.cap = pl.cap;
+    sp.np = np;
+    sp.seed = cfg->seed;
+    sp.fulcs = sc.fulcs;
     sp.child_count = counts(l);
     sp.out_lo = sc.rec_lo[out_buf(l)];
     sp.out_meta = sc.rec_meta[out_buf(l)];
     sp.status = d_status;
-    TRY_HIP(launch_split_level(sp, parents, split_per, s));
+    TRY_HIP(launch_split_level(fmt(l - 1), fmt(l), sp, parents, threads, sper, s));
   }
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[2], s));
 
@@ -416,7 +427,7 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
 
   SearchArgs ra{};
   ra.rec_lower = sc.rec_lo[0];
-  ra.rec_meta = sc.rec_meta[0];
+  ra.rec_bucket = reinterpret_cast<const uint16_t*>(sc.rec_meta[0]);
   ra.pcount = sc.pcount;
   ra.sorted = sc.rec_lo[1];  // free once partitioning is done
   ra.np = np;
@@ -1030,8 +1041,13 @@ int phobic_set_tuning(phobic_ctx* ctx, const phobic_tuning* t) {
   if (!ctx || !t) return PHOBIC_E_INVALID_CONFIG;
   const uint32_t w = t->search_waves, per = t->route_per, f = t->max_fanout;
   const uint32_t sp = t->split_per, f1 = t->first_fanout;
-  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(per == 8 || per == 12 || per == 16 || per == 20) || f < 16 || f > kMaxFanout ||
-      t->search_mode > 3 || !(sp == 0 || sp == 8 || sp == 12 || sp == 16) || (f1 != 0 && (f1 < 2 || f1 > f)))
+  const uint32_t th = t->route_threads;
+  auto tile_per = [th](uint32_t x) {
+    return th == 1024 ? (x == 4 || x == 6 || x == 8 || x == 10 || x == 12)
+                      : (x == 8 || x == 12 || x == 16 || x == 20 || x == 24);
+  };
+  if (!(w == 1 || w == 2 || w == 4 || w == 8) || !(th == 0 || th == 512 || th == 1024) || !tile_per(per) || f < 16 ||
+      f > kMaxFanout || t->search_mode > 3 || !(sp == 0 || tile_per(sp)) || (f1 != 0 && (f1 < 2 || f1 > f)))
     return PHOBIC_E_INVALID_CONFIG;
   ctx->tuning = *t;
   return PHOBIC_OK;
@@ -1085,31 +1101,37 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   };
   phobic_tuning win = saved;
   float part_ms = 1e30f, search_ms = 1e30f;
-  for (uint32_t per : {8u, 12u, 16u, 20u})
-    for (uint32_t sper : {8u, 12u, 16u})
-      for (uint32_t f1 : {0u, 16u, 32u, 48u, 64u})
-        for (uint32_t f : {64u, 128u}) {
-          if (f1 > f) continue;
-          phobic_tuning t = win;
-          t.route_per = per;
-          t.split_per = sper;
-          t.first_fanout = f1;
-          t.max_fanout = f;
-          float ms;
-          const int r = time_stages(t, 0, kStages - 2, &ms);  // hash_route + partition_split
-          if (r != PHOBIC_OK) {
-            ctx->tuning = saved;
-            cleanup();
-            return r;
-          }
-          if (ms < part_ms) {
-            part_ms = ms;
-            win.route_per = per;
-            win.split_per = sper;
-            win.first_fanout = f1;
-            win.max_fanout = f;
-          }
+  struct Shape {
+    uint32_t threads, per, sper;
+  };
+  const Shape shapes[] = {{512, 16, 16}, {512, 20, 20}, {512, 24, 20}, {512, 24, 24}, {1024, 8, 8},
+                          {1024, 10, 10}, {1024, 12, 10}, {1024, 12, 12}};
+  for (const Shape& sh : shapes)
+    for (uint32_t f1 : {0u, 48u, 512u})
+      for (uint32_t f : {128u, 1024u}) {
+        if (f1 > f) continue;
+        phobic_tuning t = win;
+        t.route_threads = sh.threads;
+        t.route_per = sh.per;
+        t.split_per = sh.sper;
+        t.first_fanout = f1;
+        t.max_fanout = f;
+        float ms;
+        const int r = time_stages(t, 0, kStages - 2, &ms);  // hash_route + partition_split
+        if (r != PHOBIC_OK) {
+          ctx->tuning = saved;
+          cleanup();
+          return r;
         }
+        if (ms < part_ms) {
+          part_ms = ms;
+          win.route_threads = sh.threads;
+          win.route_per = sh.per;
+          win.split_per = sh.sper;
+          win.first_fanout = f1;
+          win.max_fanout = f;
+        }
+      }
   for (uint32_t w : {1u, 2u, 4u, 8u})
     for (uint32_t mode : {0u, 1u, 2u, 3u}) {
       phobic_tuning t = win;
@@ -1227,6 +1249,13 @@ int phobic_check_bijection_device(phobic_ctx* ctx, const uint32_t* d_pos, uint64
   TRY_HIP(launch_fill_u32(d_bitmap, (n + 31) / 32, 0, s));
   TRY_HIP(launch_fill_u32(reinterpret_cast<uint32_t*>(d_bad), 2, 0, s));
   TRY_HIP(launch_check_bijection(d_pos, count, n, d_bitmap, d_bad, s));
+  return PHOBIC_OK;
+}
+
+int phobic_copy_device(void* d_dst, const void* d_src, uint64_t bytes, void* stream) {
+  if (bytes % 16 || (bytes && (!d_dst || !d_src)) || (((uintptr_t)d_dst | (uintptr_t)d_src) & 15))
+    return PHOBIC_E_INVALID_CONFIG;
+  TRY_HIP(launch_copy16(d_dst, d_src, bytes / 16, (hipStream_t)stream));
   return PHOBIC_OK;
 }
 

@@ -28,7 +28,14 @@ struct KeySource {
   const uint4* keys;      // kKindKey: pre-hashed 16-byte keys
 };
 
-constexpr uint32_t kMaxFanout = 128;  // child bins per partitioning level
+constexpr uint32_t kMaxFanout = 1024;  // child bins per partitioning level
+
+// Records a partitioning level writes.
+enum : int {
+  kFmtRaw = 0,    // the u64 key itself: plain u64 builds re-hash it at the next level
+  kFmtRec = 1,    // lower64 + meta32 (partition inside the bin << 16 | bucket)
+  kFmtFinal = 2,  // lower64 + bucket16: the last level, which the search reads
+};
 
 // Level 1 of the partitioning: hash keys and route them to level-1 bins.
 struct HashRouteArgs {
@@ -40,8 +47,8 @@ struct HashRouteArgs {
   uint32_t fanout;        // number of level-1 bins, <= kMaxFanout
   const uint32_t* fulcs;  // 2048 entries
   uint32_t* child_count;  // per level-1 bin, zeroed before the launch
-  uint64_t* out_lo;       // records: lower1 | lower2 << 32
-  uint32_t* out_meta;     // records: partition-in-bin << 16 | bucket
+  uint64_t* out_lo;       // raw keys, or lower1 | lower2 << 32
+  void* out_meta;         // u32 meta (kFmtRec) or u16 bucket (kFmtFinal)
   uint32_t* status;       // error bit mask
   // batched builds (sets > 0): key i of set s lies in [set_key_off[s],
   // set_key_off[s+1]); its partition is set_part_base[s] + hi32(x * np_s)
@@ -53,16 +60,21 @@ struct HashRouteArgs {
 // Levels 2..L: split every parent bin into `fanout` child bins.
 struct SplitArgs {
   const uint64_t* in_lo;
-  const uint32_t* in_meta;
+  const void* in_meta;    // u32 meta of kFmtRec input
   const uint32_t* parent_count;
   uint64_t parent_cap;    // G_parent * cap
   uint32_t slices;        // tiles per parent bin (grid = parents * slices)
   uint32_t fanout;        // children per parent
+  uint32_t G_parent;      // partitions per parent bin
   uint32_t G_child;       // partitions per child bin
   uint32_t cap;
+  // kFmtRaw input: the key is hashed again
+  uint32_t np;
+  uint64_t seed;
+  const uint32_t* fulcs;
   uint32_t* child_count;  // per child bin, zeroed before the launch
   uint64_t* out_lo;
-  uint32_t* out_meta;
+  void* out_meta;
   uint32_t* status;
 };
 
@@ -74,7 +86,7 @@ __host__ __device__ inline uint32_t table_size(uint32_t ps, uint32_t a_ppm) {
 
 struct SearchArgs {
   const uint64_t* rec_lower;
-  const uint32_t* rec_meta;  // bucket in the low 16 bits
+  const uint16_t* rec_bucket;  // bucket of each record
   const uint32_t* pcount;
   uint64_t* sorted;  // np*cap scratch: keys of each partition in bucket order
   uint32_t np, B, cap;
@@ -113,10 +125,17 @@ struct EvalArgs {
 };
 
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
-// per = records per thread of a routing tile (8 or 16; tiles of 256 * per)
-hipError_t launch_hash_route(int kind, const HashRouteArgs& a, uint32_t per, hipStream_t s);
-hipError_t launch_split_level(const SplitArgs& a, uint32_t parents, uint32_t per, hipStream_t s);
-uint32_t route_tile_records(uint32_t per);
+// Routing tiles: `threads` (512 or 1024) times `per` records per thread
+// (512: 8, 12, 16, 20 or 24; 1024: 4, 6, 8, 10 or 12); out_fmt / in_fmt are
+// kFmt* record formats.  A launch takes the largest per, at most the one
+// asked for, whose tile fits the workgroup's LDS.
+hipError_t launch_hash_route(int kind, int out_fmt, const HashRouteArgs& a, uint32_t threads, uint32_t per,
+                             hipStream_t s);
+hipError_t launch_split_level(int in_fmt, int out_fmt, const SplitArgs& a, uint32_t parents, uint32_t threads,
+                              uint32_t per, hipStream_t s);
+// records per thread a split level of `threads` uses when asked for `per`
+// (the slices per parent bin are sized for it)
+uint32_t split_tile_per(uint32_t threads, uint32_t per, int in_fmt, int out_fmt, uint32_t fanout);
 // exclusive scans of counts into offsets[np+1] and, when a_ppm > 0, of
 // table_size(count) - count into remap_offsets[np+1]; `totals` holds
 // 2 * partition_scan_tiles(np) words of scratch
@@ -133,6 +152,7 @@ hipError_t launch_batch_offsets(const uint32_t* global_offsets, const uint32_t* 
                                 uint32_t total_entries, uint32_t* out, hipStream_t s);
 hipError_t launch_check_bijection(const uint32_t* pos, uint64_t count, uint64_t n, uint32_t* bitmap,
                                   unsigned long long* bad, hipStream_t s);
+hipError_t launch_copy16(void* dst, const void* src, uint64_t words, hipStream_t s);
 hipError_t launch_splitmix64(uint64_t seed, uint64_t n, uint64_t* out, hipStream_t s);
 hipError_t launch_gen_strings(uint64_t seed, uint64_t n, const uint64_t* offs, uint8_t* bytes, hipStream_t s);
 

@@ -399,11 +399,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
 #pragma unroll
     for (uint32_t j = 0; j < kLoadBatch; j++) {
       const uint32_t i = i0 + lane + 64 * j;
-      m[j] = i < ps ? a.rec_meta[base + i] : 0xFFFFFFFFu;
+      m[j] = i < ps ? a.rec_bucket[base + i] : 0xFFFFu;  // buckets are below B <= 65535
     }
 #pragma unroll
     for (uint32_t j = 0; j < kLoadBatch; j++)
-      if (m[j] != 0xFFFFFFFFu) atomicAdd(&bcnt[m[j] & 0xFFFFu], 1u);
+      if (m[j] != 0xFFFFu) atomicAdd(&bcnt[m[j]], 1u);
   }
   lds_sync();
   bool over = false;
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
 #pragma unroll
     for (uint32_t j = 0; j < kLoadBatch; j++) {
       const uint32_t i = i0 + lane + 64 * j;
-      m[j] = i < ps ? a.rec_meta[base + i] : 0xFFFFFFFFu;
+      m[j] = i < ps ? a.rec_bucket[base + i] : 0xFFFFu;
       v[j] = i < ps ? a.rec_lower[base + i] : 0;
     }
     // All of the batch's loads, waited for at once: the vector memory
@@ -510,8 +510,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0) (gfx9 encoding)
 #pragma unroll
     for (uint32_t j = 0; j < kLoadBatch; j++) {
-      if (m[j] != 0xFFFFFFFFu) {
-        const uint32_t b = m[j] & 0xFFFFu;
+      if (m[j] != 0xFFFFu) {
+        const uint32_t b = m[j];
         sorted[atomicAdd(&bpos[b], 1u)] = v[j];
       }
     }

@@ -70,11 +70,13 @@ class Tuning(ctypes.Structure):
         ("search_mode", ctypes.c_uint32),
         ("split_per", ctypes.c_uint32),
         ("first_fanout", ctypes.c_uint32),
+        ("route_threads", ctypes.c_uint32),
     ]
 
     def as_dict(self) -> dict:
         return {"search_waves": self.search_waves, "route_per": self.route_per, "max_fanout": self.max_fanout,
-                "search_mode": self.search_mode, "split_per": self.split_per, "first_fanout": self.first_fanout}
+                "search_mode": self.search_mode, "split_per": self.split_per, "first_fanout": self.first_fanout,
+                "route_threads": self.route_threads}
 
 
 class StageTimes(ctypes.Structure):
@@ -130,6 +132,7 @@ EXPORTS = [
     "phobic_eval_pairs_device",
     "phobic_eval_keys_device",
     "phobic_check_bijection_device",
+    "phobic_copy_device",
     "phobic_gen_splitmix64_device",
     "phobic_gen_strings_device",
 ]
@@ -187,6 +190,7 @@ def load() -> ctypes.CDLL:
         "phobic_eval_pairs_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
         "phobic_eval_keys_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
         "phobic_check_bijection_device": (I, [P, P, U64, U64, P, P, P]),
+        "phobic_copy_device": (I, [P, P, U64, P]),
         "phobic_gen_splitmix64_device": (I, [U64, U64, P, P]),
         "phobic_gen_strings_device": (I, [U64, U64, P, P, P]),
     }
@@ -372,8 +376,8 @@ class Context:
         return t.as_dict()
 
     def set_tuning(self, search_waves: int, route_per: int, max_fanout: int, search_mode: int = 0,
-                   split_per: int = 0, first_fanout: int = 0) -> None:
-        t = Tuning(search_waves, route_per, max_fanout, search_mode, split_per, first_fanout)
+                   split_per: int = 0, first_fanout: int = 0, route_threads: int = 512) -> None:
+        t = Tuning(search_waves, route_per, max_fanout, search_mode, split_per, first_fanout, route_threads)
         st = self.lib.phobic_set_tuning(self.h, ctypes.byref(t))
         if st:
             raise PhobicError(st, "set_tuning")
@@ -608,6 +612,14 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def copy_device(d_dst, d_src, nbytes: int, stream=None) -> None:
+    """phobic_copy_device: the library's plain streaming copy kernel."""
+    lib = load()
+    st = lib.phobic_copy_device(_ptr(d_dst), _ptr(d_src), nbytes, _stream(stream))
+    if st:
+        raise PhobicError(st, "copy_device")
 
 
 def gen_splitmix64_device(seed: int, n: int, d_keys, stream=None) -> None:

@@ -83,7 +83,10 @@ def test_batch_single_set_equals_plain(ctx):
 
 
 def test_tuning_does_not_change_results(ctx):
-    keys = po.splitmix64(21, 600_000)
+    """Every launch shape gives the same pilots: one, two and three routing
+    levels (1465 partitions), raw-key and hashed-record levels, tiles of 512
+    and 1024 threads, and split tiles reduced to fit the LDS."""
+    keys = po.splitmix64(21, 3_000_000)
     cfg = ph.make_config(7.5, 2048)
     saved = ctx.tuning()
     ref = ctx.build_u64(keys, cfg)
@@ -91,7 +94,10 @@ def test_tuning_does_not_change_results(ctx):
         shapes = [(1, 8, 32, 1, 0, 0), (2, 16, 64, 2, 0, 0), (8, 8, 128, 3, 0, 0), (4, 16, 16, 0, 0, 0),
                   (4, 16, 128, 3, 0, 0), (2, 12, 128, 0, 0, 0), (2, 12, 40, 1, 0, 0), (2, 16, 128, 0, 8, 32),
                   (2, 16, 64, 0, 16, 16), (1, 8, 128, 0, 12, 2), (2, 16, 16, 0, 8, 3),
-                  (1, 20, 128, 0, 12, 48), (2, 20, 64, 0, 0, 32)]  # split_per 0 with 20: split tiles of 16
+                  (1, 20, 128, 0, 12, 48), (2, 20, 64, 0, 0, 32),
+                  (1, 24, 1024, 0, 24, 0), (1, 24, 1024, 0, 20, 512), (1, 20, 1024, 0, 20, 48),
+                  (1, 12, 1024, 0, 12, 0, 1024), (1, 8, 1024, 0, 4, 64, 1024), (1, 10, 40, 0, 6, 0, 1024),
+                  (1, 24, 1024, 0, 24, 1024)]  # one level of 1024 bins: 1465 partitions need a second
         for shape in shapes:
             ctx.set_tuning(*shape)
             r = ctx.build_u64(keys, cfg)
@@ -106,17 +112,17 @@ def test_tuning_defaults_and_validation():
     c = ph.Context(0)
     try:
         d = c.tuning()
-        assert d == {"search_waves": 1, "route_per": 20, "max_fanout": 128, "search_mode": 0, "split_per": 12,
-                     "first_fanout": 48}
-        for bad in ({"search_waves": 3}, {"route_per": 24}, {"route_per": 10}, {"split_per": 20},
-                    {"max_fanout": 129}, {"max_fanout": 8}, {"search_mode": 4}, {"first_fanout": 1},
-                    {"first_fanout": 200}):
+        assert d == {"search_waves": 1, "route_per": 12, "max_fanout": 1024, "search_mode": 0, "split_per": 12,
+                     "first_fanout": 0, "route_threads": 1024}
+        for bad in ({"search_waves": 3}, {"route_per": 28}, {"route_per": 14}, {"split_per": 16},
+                    {"max_fanout": 1025}, {"max_fanout": 8}, {"search_mode": 4}, {"first_fanout": 1},
+                    {"first_fanout": 2000}, {"route_threads": 256}, {"route_per": 24}):
             t = dict(d)
             t.update(bad)
             with pytest.raises(ph.PhobicError):
                 c.set_tuning(**t)
             assert c.tuning() == d, bad
-        c.set_tuning(2, 16, 64, 1, 8, 32)
+        c.set_tuning(2, 16, 64, 1, 8, 32, 512)
         assert c.tuning()["route_per"] == 16 and c.tuning()["split_per"] == 8
     finally:
         c.close()
@@ -125,8 +131,9 @@ def test_tuning_defaults_and_validation():
 def test_autotune_returns_valid_shape(ctx):
     saved = ctx.tuning()
     best, ms = ctx.autotune(ph.make_config(7.0, 2048), 2_000_000)
-    assert best["search_waves"] in (1, 2, 4, 8) and best["route_per"] in (8, 12, 16, 20)
-    assert best["max_fanout"] in (64, 128) and best["search_mode"] in (0, 1, 2, 3) and ms > 0
-    assert best["split_per"] in (8, 12, 16) and best["first_fanout"] in (0, 16, 32, 48, 64)
+    assert best["search_waves"] in (1, 2, 4, 8) and best["route_threads"] in (512, 1024)
+    assert best["route_per"] in ((16, 20, 24) if best["route_threads"] == 512 else (8, 10, 12))
+    assert best["max_fanout"] in (128, 1024) and best["search_mode"] in (0, 1, 2, 3) and ms > 0
+    assert best["split_per"] in (8, 10, 12, 16, 20, 24) and best["first_fanout"] in (0, 48, 512)
     assert ctx.tuning() == best
     ctx.set_tuning(**saved)

@@ -358,7 +358,7 @@ def test_search_modes_bit_exact(ctx, n, lam, P, alpha, kseed):
     saved = ctx.tuning()
     try:
         for mode in (0, 1, 2, 3):
-            ctx.set_tuning(saved["search_waves"], saved["route_per"], saved["max_fanout"], mode)
+            ctx.set_tuning(**{**saved, "search_mode": mode})
             r = ctx.build_u64(keys, cfg)
             assert np.array_equal(r.pilots, opil), mode
             assert np.array_equal(r.offsets, ooff), mode

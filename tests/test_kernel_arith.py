@@ -7,7 +7,8 @@ and v_mul_f32, to nearest even; v_cvt_u32_f32 truncates like astype):
 - search_kernel.hip mod_f32: a mod d for a < 2^31, 512 <= d < 2^24, as the
   smallest of r, r + d, r - d with r = a - q*d (mod 2^32);
 - partition_kernels.hip k_hash_route: the level-1 bin p / G and p mod G for
-  G < 2^16 and p / G < 2^8, with one correction step.
+  G < 2^16, p < 2^21 partitions and p / G at most 1024 bins, with one
+  correction step.
 
 Both are compared with exact integer division, near every quotient boundary
 and on random points.
@@ -61,10 +62,11 @@ def test_level1_bin_exact():
     rng = np.random.default_rng(12)
     Gs = np.unique(np.concatenate([np.arange(1, 300), rng.integers(300, 1 << 16, 700), [15376, 65535]]))
     for G in Gs.tolist():
-        c = np.arange(0, 129, dtype=np.int64)
+        cmax = min(1025, (1 << 21) // G + 1)  # at most 1024 bins, partitions below 2^21
+        c = np.arange(0, cmax, dtype=np.int64)
         p = np.concatenate([((c * G)[:, None] + np.arange(-3, 4)[None, :]).ravel(),
-                            ((c * G)[:, None] + rng.integers(0, G, (129, 8))).ravel()])
-        p = p[(p >= 0) & (p < 129 * G)]
+                            ((c * G)[:, None] + rng.integers(0, G, (cmax, 8))).ravel()])
+        p = p[(p >= 0) & (p < min(cmax * G, 1 << 21))]
         got_c, got_r = bin_f32(p, G)
         assert np.array_equal(got_c, p // G), G
         assert np.array_equal(got_r, p % G), G
