@@ -80,15 +80,29 @@ typedef struct {
 
 /* Device time of each stage in milliseconds (HIP events around each
  * launch), playing the role of the reference's GPU_* timestamp labels
- * (mphf_builder.h:138-146).  names[i] labels ms[i]; the stage split is this
- * library's own (hashing moved on device, bucket sort + redistribution +
- * search fused per partition), so the labels differ from the reference's. */
+ * (mphf_builder.h:138-146, 229-290).  names[i] labels ms[i] in launch order:
+ *   hash_route         hash, partition and bucket of every key, level-1 routing
+ *   partition_split    routing levels 2..L: keys to their partitions
+ *   partition_offsets  exclusive scan of the partition key counts
+ *   partition_search   per partition: bucket sizes, bucket sort, keys in
+ *                      bucket order, pilot search (and the alpha < 1 remap)
+ * The reference's labels map onto these as: GPU_bucket_sizes = hash_route,
+ * GPU_key_redistribution = partition_split, GPU_partition_offsets =
+ * partition_offsets, GPU_search = partition_search, and GPU_bucket_sorting,
+ * GPU_apply_partition_offsets and GPU_memory_map = 0 (fused into
+ * partition_search, not needed, outputs stay in device memory); the C++
+ * MPHFbuilder::build emits the reference's labels.  The host-buffer entry
+ * points also time their copies: input_transfer_ms from the first input
+ * copy to the first kernel, result_transfer_ms from the last kernel's end to
+ * the outputs in host memory (0 for the device entry points). */
 #define PHOBIC_MAX_STAGES 8
 typedef struct {
   uint32_t count;
   float ms[PHOBIC_MAX_STAGES];
   char names[PHOBIC_MAX_STAGES][32];
   float total_ms; /* first launch start to last launch end */
+  float input_transfer_ms;
+  float result_transfer_ms;
 } phobic_stage_times;
 
 typedef struct phobic_ctx phobic_ctx;
@@ -188,39 +202,55 @@ int phobic_batch_graph_launch(phobic_batch* batch, const uint64_t* d_keys, uint3
 int phobic_batch_destroy(phobic_batch* batch);
 
 /* ---- serving: continuous-batching build server ----
- * Requests are admitted in arrival order (a later request may overtake one
- * that does not fit yet) whenever one of `slots` workers is idle and the
- * request's device-memory estimate fits `hbm_budget`; each worker owns a
- * context and stream, so admitted builds run concurrently on the device.
- * Results stay in device memory until phobic_server_release.  Host key
- * buffers must stay valid until the request completes.  No counterpart in
- * the reference (one synchronous MPHFbuilder::build per MPHF). */
+ * Requests queue in arrival order.  Whenever one of `slots` workers is idle,
+ * the scheduler takes the oldest request that fits the HBM budget (a later
+ * one may overtake a request that does not fit yet) and, when it is small
+ * (at most 4e6 keys: u64 keys, or byte strings in host memory, with the
+ * default OptBucketer table), every queued small request of the same
+ * configuration (key kind, lambda, P, seed, alpha) with it, up to 64e6 keys
+ * and 1024 requests: they are built together as one flattened launch whose
+ * per-request outputs equal building each alone.  Larger requests run on a
+ * slot of their own.  Each worker owns a context and stream, so admitted
+ * launches run concurrently on the device.  Results stay in device memory
+ * until phobic_server_release.  Host key buffers must stay valid until the
+ * request completes.
+ * Memory: hbm_in_use counts the outputs kept for unreleased requests, the
+ * staging, outputs and scratch growth charged to running launches, and the
+ * scratch each worker's context keeps between launches (reused by its next
+ * launch), all in the library pool's 2 MiB units; admission keeps it within
+ * hbm_budget.  A request whose own need exceeds the budget is refused at
+ * submit.  No counterpart in the reference (one synchronous
+ * MPHFbuilder::build per MPHF). */
 #define PHOBIC_KEYS_U64 0
 #define PHOBIC_KEYS_BYTES 1
 typedef struct {
   int kind;                 /* PHOBIC_KEYS_U64 or PHOBIC_KEYS_BYTES */
   int keys_on_device;       /* 0: host pointers, 1: device pointers */
   const void* keys;         /* u64[n] or the concatenated bytes */
-  const uint64_t* offsets;  /* bytes: n+1 offsets into keys */
-  uint64_t bytes_len;       /* bytes: offsets[n] */
+  const uint64_t* offsets;  /* bytes: n+1 non-decreasing offsets into keys */
+  uint64_t bytes_len;       /* bytes: at least offsets[n] */
   uint64_t n;
   phobic_config cfg;
 } phobic_request;
 typedef struct {
-  uint32_t slots;           /* concurrent builds (1..64) */
+  uint32_t slots;           /* concurrent launches (1..64) */
   int device;
-  uint64_t hbm_budget;      /* device bytes requests may hold */
+  uint64_t hbm_budget;      /* device bytes the server may hold */
   uint32_t queue_capacity;  /* 0 = 65536 */
 } phobic_server_config;
 typedef struct {
   uint64_t submitted, completed, rejected, keys_built, hbm_in_use;
   double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion latency */
   double span_s;                          /* first submit -> last completion */
+  uint64_t grouped_launches;              /* launches that built several requests */
+  uint64_t grouped_requests;              /* requests built in such launches */
+  uint64_t pool_used;                     /* bytes the library's device pool has handed out */
 } phobic_server_stats;
 typedef struct phobic_server phobic_server;
 int phobic_server_create(const phobic_server_config* cfg, phobic_server** out);
 /* PHOBIC_E_BUSY when the queue is full, PHOBIC_E_OOM when the request alone
- * exceeds the budget. */
+ * exceeds the budget, PHOBIC_E_INVALID_CONFIG for a malformed request (host
+ * string offsets must be non-decreasing and end within bytes_len). */
 int phobic_server_submit(phobic_server* server, const phobic_request* req, uint64_t* id);
 int phobic_server_poll(phobic_server* server, uint64_t id, int* done);
 int phobic_server_wait(phobic_server* server, uint64_t id, int* status, double* latency_ms);
@@ -320,10 +350,10 @@ int phobic_copy_device(void* d_dst, const void* d_src, uint64_t bytes, void* str
  * (state += 0x9E3779B97F4A7C15, then the 30/27/31 mix).  Distinct by
  * construction. */
 int phobic_gen_splitmix64_device(uint64_t seed, uint64_t n, uint64_t* d_keys, void* stream);
-/* String i fills d_bytes[d_offsets[i], d_offsets[i+1]) (every length >= 8):
- * its first 8 bytes are the hex digits of i's low 32 bits (so up to 2^32
- * strings are distinct), the rest printable ASCII 0x20..0x7E from a
- * splitmix64 stream seeded by (seed, i). */
+/* String i fills d_bytes[d_offsets[i], d_offsets[i+1]) with printable ASCII
+ * 0x20..0x7E drawn from a splitmix64 stream seeded by (seed, i).  Strings
+ * are not made distinct: duplicates are unlikely at lengths of 8 and more,
+ * and a build that meets one returns PHOBIC_E_DUPLICATE_KEYS. */
 int phobic_gen_strings_device(uint64_t seed, uint64_t n, const uint64_t* d_offsets, uint8_t* d_bytes, void* stream);
 
 #ifdef __cplusplus

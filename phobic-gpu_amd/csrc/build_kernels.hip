@@ -230,6 +230,23 @@ __global__ __launch_bounds__(256) void k_copy16(uint4* __restrict__ dst, const u
   for (; i < n; i += stride) dst[i] = src[i];
 }
 
+__global__ __launch_bounds__(256) void k_batch_remap_bases(const uint32_t* __restrict__ goff,
+                                                           const uint32_t* __restrict__ pb,
+                                                           const uint32_t* __restrict__ rbase, uint32_t sets,
+                                                           uint32_t np, uint32_t* __restrict__ out) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  // the set s with pb[s] <= p < pb[s+1] (empty sets share their base)
+  uint32_t lo = 0, hi = sets;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (pb[mid] <= p) lo = mid;
+    else hi = mid;
+  }
+  while (lo + 1 < sets && pb[lo + 1] <= p) lo++;
+  out[p] = rbase[lo] + goff[p] - goff[pb[lo]];
+}
+
 // splitmix64 (Steele, Lea, Flood 2014): output i uses state seed + (i+1)*gamma.
 __global__ __launch_bounds__(256) void k_splitmix64(uint64_t seed, uint64_t n, uint64_t* out) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -242,9 +259,10 @@ __global__ __launch_bounds__(256) void k_splitmix64(uint64_t seed, uint64_t n, u
   }
 }
 
-// Synthetic distinct printable strings: string i fills [offs[i], offs[i+1])
-// (lengths >= 8); its first 8 bytes are the hex digits of i's low 32 bits,
-// the rest printable ASCII 0x20..0x7E from a splitmix64 stream of (seed, i).
+// Synthetic printable strings: string i fills [offs[i], offs[i+1]) with bytes
+// 0x20..0x7E drawn from a splitmix64 stream of (seed, i), eight per output.
+// Nothing makes two strings differ; at lengths of 8 and more, duplicates in
+// a few 1e8 strings are unlikely, and a build reports any it meets.
 __global__ __launch_bounds__(256) void k_gen_strings(uint64_t seed, uint64_t n, const uint64_t* __restrict__ offs,
                                                     uint8_t* __restrict__ bytes) {
   uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -254,22 +272,15 @@ __global__ __launch_bounds__(256) void k_gen_strings(uint64_t seed, uint64_t n, 
     uint64_t state = seed ^ (i * 0xD1B54A32D192ED03ull);
     uint64_t r = 0;
     for (uint64_t j = 0; j < len; j++) {
-      uint8_t c;
-      if (j < 8) {
-        const uint32_t d = (uint32_t)(i >> (4 * j)) & 15u;
-        c = (uint8_t)(d < 10 ? '0' + d : 'a' + d - 10);
-      } else {
-        if ((j - 8) % 8 == 0) {
-          state += 0x9E3779B97F4A7C15ull;
-          uint64_t z = state;
-          z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-          z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-          r = z ^ (z >> 31);
-        }
-        c = (uint8_t)(0x20 + (uint32_t)((r & 0xFF) % 95u));
-        r >>= 8;
+      if (j % 8 == 0) {
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        r = z ^ (z >> 31);
       }
-      bytes[a + j] = c;
+      bytes[a + j] = (uint8_t)(0x20 + (uint32_t)((r & 0xFF) % 95u));
+      r >>= 8;
     }
   }
 }
@@ -321,6 +332,15 @@ hipError_t launch_batch_offsets(const uint32_t* global_offsets, const uint32_t* 
   if (total_entries == 0) return hipSuccess;
   hipLaunchKernelGGL(k_batch_offsets, dim3((total_entries + 255) / 256), dim3(256), 0, s, global_offsets,
                      set_part_base, sets, total_entries, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_remap_bases(const uint32_t* goff, const uint32_t* set_part_base,
+                                    const uint32_t* set_remap_base, uint32_t sets, uint32_t np, uint32_t* out,
+                                    hipStream_t s) {
+  if (np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_remap_bases, dim3((np + 255) / 256), dim3(256), 0, s, goff, set_part_base,
+                     set_remap_base, sets, np, out);
   return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -98,6 +99,8 @@ struct phobic_ctx {
   hipStream_t stream = nullptr;
   // scratch owned by the context
   DevBuf pcount, lvl_count, scan_tot, rec_lo[2], rec_meta[2], fulcs, status;
+  // tables of a group build (several key sets in one launch)
+  DevBuf grp;
   // staging for host-buffer entry points
   DevBuf in_a, in_b, out_pilots, out_offsets, out_remap, out_roff;
   // two pinned host chunks for large host <-> device copies (allocated on
@@ -105,7 +108,9 @@ struct phobic_ctx {
   void* pinned[2] = {};
   hipEvent_t pin_ev[2] = {};
   hipEvent_t ev[kStages + 1] = {};
+  hipEvent_t ev_in = nullptr;  // host-buffer builds: before the first input copy
   bool timed = false;
+  float in_ms = 0, out_ms = 0;  // copies of the last host-buffer build
   phobic_tuning tuning{1, 12, kMaxFanout, 0, 12, 0, 1024};
   // fulcrums currently in ctx->fulcs
   uint32_t f_host[kFulcsTable];
@@ -249,6 +254,12 @@ struct BatchInfo {
   const uint64_t* set_key_off = nullptr;
   const uint32_t* set_part_base = nullptr;
   uint32_t* global_offsets = nullptr;  // np + 1 scratch for the per-set offsets
+  // alpha < 1: the global remap offsets and each partition's remap start in
+  // the output (np + 1 scratch each), and where set s's remap entries start
+  // in the remap output (sets entries)
+  uint32_t* global_roff = nullptr;
+  uint32_t* part_remap = nullptr;
+  const uint32_t* set_remap_base = nullptr;
 };
 
 // Layout of the context scratch for a plan.
@@ -415,10 +426,17 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[2], s));
 
   uint32_t* totals = sc.scan_tot;
-  if (batch.sets) {  // batches build at alpha = 1
-    TRY_HIP(launch_partition_offsets(sc.pcount, np, 0, batch.global_offsets, nullptr, totals, s));
+  if (batch.sets) {  // global scans, then every set's own offsets
+    TRY_HIP(launch_partition_offsets(sc.pcount, np, remap ? pl.a_ppm : 0, batch.global_offsets,
+                                     remap ? batch.global_roff : nullptr, totals, s));
     TRY_HIP(launch_batch_offsets(batch.global_offsets, batch.set_part_base, batch.sets, np + batch.sets,
                                  d_out.offsets, s));
+    if (remap) {
+      TRY_HIP(launch_batch_offsets(batch.global_roff, batch.set_part_base, batch.sets, np + batch.sets,
+                                   d_out.remap_offsets, s));
+      TRY_HIP(launch_batch_remap_bases(batch.global_roff, batch.set_part_base, batch.set_remap_base, batch.sets, np,
+                                       batch.part_remap, s));
+    }
   } else {
     TRY_HIP(launch_partition_offsets(sc.pcount, np, remap ? pl.a_ppm : 0, d_out.offsets,
                                      remap ? d_out.remap_offsets : nullptr, totals, s));
@@ -437,7 +455,7 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
   ra.tcap = pl.tcap;
   ra.pilots = d_pilots;
   ra.remap = remap ? d_out.remap : nullptr;
-  ra.remap_offsets = remap ? d_out.remap_offsets : nullptr;
+  ra.remap_offsets = remap ? (batch.sets ? batch.part_remap : d_out.remap_offsets) : nullptr;
   ra.status = d_status;
   ra.sets = batch.sets;
   ra.set_part_base = batch.set_part_base;
@@ -476,6 +494,8 @@ int fill_times(phobic_ctx* ctx, phobic_stage_times* t) {
     std::snprintf(t->names[i], sizeof(t->names[i]), "%s", kStageNames[i]);
   }
   TRY_HIP(hipEventElapsedTime(&t->total_ms, ctx->ev[0], ctx->ev[kStages]));
+  t->input_transfer_ms = ctx->in_ms;
+  t->result_transfer_ms = ctx->out_ms;
   return PHOBIC_OK;
 }
 
@@ -585,6 +605,7 @@ int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const v
   TRY(set_device(ctx));
   hipStream_t s = ctx->stream;
   TRY_HIP(ctx->in_a.ensure(a_bytes, s));
+  TRY_HIP(hipEventRecord(ctx->ev_in, s));
   if (a_bytes) TRY(copy_to_device(ctx, ctx->in_a.ptr, a, a_bytes, s));
   if (b) {
     TRY_HIP(ctx->in_b.ensure(b_bytes, s));
@@ -609,6 +630,10 @@ int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const v
   phobic_outputs d_out{ctx->out_pilots.as<uint32_t>(), ctx->out_offsets.as<uint32_t>(),
                        remap ? ctx->out_remap.as<uint32_t>() : nullptr, remap ? ctx->out_roff.as<uint32_t>() : nullptr};
   TRY(enqueue_build(ctx, kind, src, cfg, pl, d_out, d_status, s));
+  // the copies back cannot start before the last kernel ends: waiting for
+  // it first separates the result transfer from the device time
+  TRY_HIP(hipEventSynchronize(ctx->ev[kStages]));
+  const auto t_out = std::chrono::steady_clock::now();
   uint32_t bits = 0;
   TRY_HIP(hipMemcpyAsync(&bits, d_status, 4, hipMemcpyDeviceToHost, s));
   if (NB) TRY(copy_to_host(ctx, out->pilots, ctx->out_pilots.ptr, NB * 4, s));
@@ -617,6 +642,8 @@ int host_build(phobic_ctx* ctx, int kind, const void* a, size_t a_bytes, const v
   TRY_HIP(hipStreamSynchronize(s));
   if (remap && out->remap_offsets[pl.np])
     TRY(copy_to_host(ctx, out->remap, ctx->out_remap.ptr, (uint64_t)out->remap_offsets[pl.np] * 4, s));
+  ctx->out_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_out).count();
+  TRY_HIP(hipEventElapsedTime(&ctx->in_ms, ctx->ev_in, ctx->ev[0]));
   TRY(fill_times(ctx, times));
   return status_from_bits(bits);
 }
@@ -628,12 +655,192 @@ int device_build(phobic_ctx* ctx, int kind, const KeySource& src, uint64_t n, co
   TRY(validate(cfg, n, &pl));
   if (!outputs_ok(d_out, pl)) return PHOBIC_E_INVALID_CONFIG;
   TRY(set_device(ctx));
+  ctx->in_ms = ctx->out_ms = 0;
   return enqueue_build(ctx, kind, src, cfg, pl, *d_out, d_status, (hipStream_t)stream);
 }
 
 }  // namespace
 
 hipStream_t phobic::ctx_stream(phobic_ctx* ctx) { return ctx->stream; }
+
+namespace {
+
+constexpr uint64_t kPoolGrain = 2ull << 20;  // allocations are counted in whole 2 MiB units
+uint64_t grain(uint64_t b) { return (b + kPoolGrain - 1) / kPoolGrain * kPoolGrain; }
+
+// Byte offsets of a group build's tables inside ctx->grp.
+struct GroupTables {
+  uint64_t key_off, part_base, remap_base, goff, groff, part_remap, bytes;
+};
+GroupTables group_tables(uint32_t sets, uint32_t np) {
+  GroupTables t;
+  uint64_t at = 0;
+  auto take = [&at](uint64_t b) {
+    const uint64_t r = at;
+    at += (b + 255) & ~uint64_t(255);
+    return r;
+  };
+  t.key_off = take(8ull * (sets + 1));
+  t.part_base = take(4ull * (sets + 1));
+  t.remap_base = take(4ull * (sets + 1));
+  t.goff = take(4ull * (np + 1));
+  t.groff = take(4ull * (np + 1));
+  t.part_remap = take(4ull * (np + 1));
+  t.bytes = at;
+  return t;
+}
+
+// The flattened plan of a group: np = the sets' partitions together.
+int group_plan(const phobic_config* cfg, const phobic::GroupLayout& gl, Plan* pl) {
+  TRY(validate(cfg, 0, pl));
+  if (gl.n > 0xFFFFFFFFull || gl.np > (1u << 21)) return PHOBIC_E_INVALID_CONFIG;
+  pl->n = gl.n;
+  pl->np = gl.np;
+  if (!plan_levels(pl)) return PHOBIC_E_INVALID_CONFIG;
+  return PHOBIC_OK;
+}
+
+// Bytes each scratch buffer of the context must hold for a plan, in the
+// order of scratch_bufs.
+void scratch_needs(const phobic_ctx* ctx, const Plan& pl0, uint64_t grp_bytes, uint64_t (&need)[9]) {
+  const ScratchSizes z = scratch_sizes(tuned(ctx, pl0));
+  const uint64_t v[9] = {z.pcount, z.lvl_count, z.scan_tot, z.rec_lo, z.rec_lo, z.rec_meta, z.rec_meta, z.fulcs,
+                         grp_bytes};
+  for (int i = 0; i < 9; i++) need[i] = v[i];
+}
+void scratch_bufs(const phobic_ctx* ctx, const DevBuf* (&b)[9]) {
+  const DevBuf* v[9] = {&ctx->pcount, &ctx->lvl_count, &ctx->scan_tot, &ctx->rec_lo[0], &ctx->rec_lo[1],
+                        &ctx->rec_meta[0], &ctx->rec_meta[1], &ctx->fulcs, &ctx->grp};
+  for (int i = 0; i < 9; i++) b[i] = v[i];
+}
+
+// from_empty: as if the context held no scratch yet
+uint64_t growth(const phobic_ctx* ctx, const Plan& pl, uint64_t grp_bytes, bool from_empty) {
+  uint64_t need[9];
+  scratch_needs(ctx, pl, grp_bytes, need);
+  const DevBuf* b[9];
+  scratch_bufs(ctx, b);
+  uint64_t g = 0;
+  for (int i = 0; i < 9; i++) {
+    const uint64_t cap = from_empty ? 0 : b[i]->cap;
+    if (need[i] > cap) g += grain(need[i]) - grain(cap);
+  }
+  return g;
+}
+
+}  // namespace
+
+int phobic::group_layout(const phobic_config* cfg, const uint64_t* set_sizes, uint32_t sets, GroupLayout* gl) {
+  if (!cfg || !gl || (sets && !set_sizes)) return PHOBIC_E_INVALID_CONFIG;
+  Plan one;
+  TRY(validate(cfg, 0, &one));
+  try {
+    gl->key_off.assign(sets + 1, 0);
+    gl->part_base.assign(sets + 1, 0);
+    gl->remap_base.assign(sets + 1, 0);
+  } catch (...) {
+    return PHOBIC_E_OOM;
+  }
+  gl->sets = sets;
+  gl->B = one.B;
+  uint64_t parts = 0, rem = 0;
+  for (uint32_t s = 0; s < sets; s++) {
+    Plan ps;
+    TRY(validate(cfg, set_sizes[s], &ps));
+    gl->key_off[s + 1] = gl->key_off[s] + set_sizes[s];
+    parts += ps.np;
+    rem += remap_capacity(ps);
+    if (parts > (1u << 21) || rem > 0xFFFFFFFFull || gl->key_off[s + 1] > 0xFFFFFFFFull)
+      return PHOBIC_E_INVALID_CONFIG;
+    gl->part_base[s + 1] = (uint32_t)parts;
+    gl->remap_base[s + 1] = (uint32_t)rem;
+  }
+  gl->np = (uint32_t)parts;
+  gl->n = gl->key_off[sets];
+  gl->a_ppm = one.a_ppm;
+  return PHOBIC_OK;
+}
+
+uint64_t phobic::group_scratch_growth(const phobic_ctx* ctx, const phobic_config* cfg, const GroupLayout& gl,
+                                      bool from_empty) {
+  Plan pl;
+  if (group_plan(cfg, gl, &pl) != PHOBIC_OK) return 0;
+  return growth(ctx, pl, group_tables(gl.sets, gl.np).bytes, from_empty);
+}
+
+uint64_t phobic::build_scratch_growth(const phobic_ctx* ctx, const phobic_config* cfg, uint64_t n, bool from_empty) {
+  Plan pl;
+  if (validate(cfg, n, &pl) != PHOBIC_OK) return 0;
+  return growth(ctx, pl, 0, from_empty);
+}
+
+void phobic::ctx_trim(phobic_ctx* ctx) {
+  (void)hipSetDevice(ctx->device);
+  for (DevBuf* b : {&ctx->pcount, &ctx->lvl_count, &ctx->scan_tot, &ctx->rec_lo[0], &ctx->rec_lo[1], &ctx->rec_meta[0],
+                    &ctx->rec_meta[1], &ctx->fulcs, &ctx->grp, &ctx->status, &ctx->in_a, &ctx->in_b, &ctx->out_pilots,
+                    &ctx->out_offsets, &ctx->out_remap, &ctx->out_roff})
+    b->release(ctx->stream);
+  std::memset(ctx->f_host, 0, sizeof(ctx->f_host));
+  (void)hipStreamSynchronize(ctx->stream);
+}
+
+uint64_t phobic::ctx_scratch_bytes(const phobic_ctx* ctx) {
+  const DevBuf* b[9];
+  scratch_bufs(ctx, b);
+  uint64_t t = 0;
+  for (const DevBuf* x : b) t += grain(x->cap);
+  for (const DevBuf* x : {&ctx->status, &ctx->in_a, &ctx->in_b, &ctx->out_pilots, &ctx->out_offsets, &ctx->out_remap,
+                          &ctx->out_roff})
+    t += grain(x->cap);
+  return t;
+}
+
+uint64_t phobic::pool_bytes(uint64_t b) { return grain(b); }
+
+uint64_t phobic::pool_used_bytes() {
+  hipMemPool_t pool = nullptr;
+  if (pool_of_current(&pool) != hipSuccess) return 0;
+  uint64_t v = 0;
+  if (hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &v) != hipSuccess) return 0;
+  return v;
+}
+
+int phobic::group_build(phobic_ctx* ctx, const phobic_config* cfg, int kind, const void* d_keys,
+                        const uint64_t* d_offs, const GroupLayout& gl, const phobic_outputs& block, uint32_t* d_status,
+                        hipStream_t s) {
+  if (!ctx || !d_status || gl.sets == 0 || (kind != kKindU64 && kind != kKindBytes)) return PHOBIC_E_INVALID_CONFIG;
+  if (gl.n && (!d_keys || (kind == kKindBytes && !d_offs))) return PHOBIC_E_INVALID_CONFIG;
+  Plan pl;
+  TRY(group_plan(cfg, gl, &pl));
+  const bool remap = pl.a_ppm < 1000000u;
+  if (!block.pilots || !block.offsets || (remap && (!block.remap || !block.remap_offsets))) return PHOBIC_E_INVALID_CONFIG;
+  TRY(set_device(ctx));
+  const GroupTables t = group_tables(gl.sets, gl.np);
+  TRY_HIP(ctx->grp.ensure(t.bytes, s));
+  char* g = ctx->grp.as<char>();
+  // from pageable memory: each copy has read its source when it returns
+  TRY_HIP(hipMemcpyAsync(g + t.key_off, gl.key_off.data(), 8ull * (gl.sets + 1), hipMemcpyHostToDevice, s));
+  TRY_HIP(hipMemcpyAsync(g + t.part_base, gl.part_base.data(), 4ull * (gl.sets + 1), hipMemcpyHostToDevice, s));
+  TRY_HIP(hipMemcpyAsync(g + t.remap_base, gl.remap_base.data(), 4ull * (gl.sets + 1), hipMemcpyHostToDevice, s));
+  TRY(prepare_build(ctx, cfg, pl, s));
+  BatchInfo bi;
+  bi.sets = gl.sets;
+  bi.set_key_off = reinterpret_cast<const uint64_t*>(g + t.key_off);
+  bi.set_part_base = reinterpret_cast<const uint32_t*>(g + t.part_base);
+  bi.set_remap_base = reinterpret_cast<const uint32_t*>(g + t.remap_base);
+  bi.global_offsets = reinterpret_cast<uint32_t*>(g + t.goff);
+  bi.global_roff = reinterpret_cast<uint32_t*>(g + t.groff);
+  bi.part_remap = reinterpret_cast<uint32_t*>(g + t.part_remap);
+  KeySource src{};
+  if (kind == kKindU64) {
+    src.u64 = static_cast<const uint64_t*>(d_keys);
+  } else {
+    src.bytes = static_cast<const uint8_t*>(d_keys);
+    src.offs = d_offs;
+  }
+  ctx->in_ms = ctx->out_ms = 0;
+  return launch_build(ctx, ctx_scratch(ctx), kind, src, cfg, pl, block, d_status, s, bi, true);
+}
 
 int phobic::ctx_device(const phobic_ctx* ctx) { return ctx->device; }
 
@@ -676,6 +883,7 @@ int phobic::build_u64_in_scratch(phobic_ctx* ctx, const uint64_t* d_keys, uint64
   }
   KeySource src{};
   src.u64 = d_keys;
+  ctx->in_ms = ctx->out_ms = 0;
   return launch_build(ctx, sc, kKindU64, src, cfg, pl0, *d_out, d_status, s, BatchInfo{}, true);
 }
 
@@ -912,6 +1120,7 @@ int phobic_ctx_create(int device, phobic_ctx** out) {
   }
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   for (int i = 0; e == hipSuccess && i <= kStages; i++) e = hipEventCreate(&ctx->ev[i]);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev_in);
   if (e != hipSuccess) {
     phobic_ctx_destroy(ctx);
     return hip_status(e);
@@ -926,7 +1135,7 @@ int phobic_ctx_destroy(phobic_ctx* ctx) {
   // device builds may have run on caller streams
   (void)hipDeviceSynchronize();
   for (DevBuf* b : {&ctx->pcount, &ctx->lvl_count, &ctx->scan_tot, &ctx->rec_lo[0], &ctx->rec_lo[1], &ctx->rec_meta[0],
-                    &ctx->rec_meta[1], &ctx->fulcs, &ctx->status, &ctx->in_a, &ctx->in_b, &ctx->out_pilots,
+                    &ctx->rec_meta[1], &ctx->fulcs, &ctx->grp, &ctx->status, &ctx->in_a, &ctx->in_b, &ctx->out_pilots,
                     &ctx->out_offsets, &ctx->out_remap, &ctx->out_roff})
     b->release(ctx->stream);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
@@ -937,6 +1146,7 @@ int phobic_ctx_destroy(phobic_ctx* ctx) {
   }
   for (int i = 0; i <= kStages; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PHOBIC_OK;
@@ -1084,6 +1294,7 @@ int phobic_autotune(phobic_ctx* ctx, const phobic_config* cfg, uint64_t sample_k
   phobic_outputs outs{pil.as<uint32_t>(), off.as<uint32_t>(), rcap ? rm.as<uint32_t>() : nullptr,
                       rcap ? ro.as<uint32_t>() : nullptr};
   const phobic_tuning saved = ctx->tuning;
+  ctx->in_ms = ctx->out_ms = 0;
   // The partitioning shape (route_per, max_fanout) only changes the stages
   // before the offsets and the search shape (search_waves, search_mode) only
   // the search, so each is tuned on its own stages' device time.

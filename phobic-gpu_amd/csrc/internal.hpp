@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/phobic_hip.h"
 
 namespace phobic {
@@ -9,6 +11,42 @@ namespace phobic {
 // The stream a context's host-buffer builds run on.
 hipStream_t ctx_stream(phobic_ctx* ctx);
 int ctx_device(const phobic_ctx* ctx);
+
+// A group: several independent key sets of one configuration built as one
+// flattened launch (every set's outputs equal building it alone).  Set s
+// holds keys [key_off[s], key_off[s+1]) of the group's key array; its
+// outputs in the group's output block are
+//   pilots         block.pilots + B * part_base[s]     (np_s * B)
+//   offsets        block.offsets + part_base[s] + s    (np_s + 1)
+//   remap_offsets  block.remap_offsets + part_base[s] + s (np_s + 1, alpha < 1)
+//   remap          block.remap + remap_base[s]          (alpha < 1)
+// with block.pilots holding B * np entries, block.offsets and
+// block.remap_offsets np + sets, block.remap remap_base[sets].
+struct GroupLayout {
+  uint32_t sets = 0, np = 0, B = 0, a_ppm = 1000000;
+  uint64_t n = 0;
+  std::vector<uint64_t> key_off;
+  std::vector<uint32_t> part_base, remap_base;
+};
+int group_layout(const phobic_config* cfg, const uint64_t* set_sizes, uint32_t sets, GroupLayout* gl);
+// Enqueues the group's build on s (kind: u64 keys, or bytes with d_offs the
+// group's n + 1 string offsets into d_keys).
+int group_build(phobic_ctx* ctx, const phobic_config* cfg, int kind, const void* d_keys, const uint64_t* d_offs,
+                const GroupLayout& gl, const phobic_outputs& block, uint32_t* d_status, hipStream_t s);
+// Device bytes the context's scratch would grow by for a build of n keys /
+// a group (0 when it already holds enough; from_empty: as if it held
+// none), counted in the pool's units.
+uint64_t build_scratch_growth(const phobic_ctx* ctx, const phobic_config* cfg, uint64_t n, bool from_empty = false);
+uint64_t group_scratch_growth(const phobic_ctx* ctx, const phobic_config* cfg, const GroupLayout& gl,
+                              bool from_empty = false);
+// Device bytes the context holds (scratch and staging), in the pool's units,
+// and their release (no build may be running on the context).
+uint64_t ctx_scratch_bytes(const phobic_ctx* ctx);
+void ctx_trim(phobic_ctx* ctx);
+// An allocation of b bytes in the pool's units, and the bytes the current
+// device's pool has handed out.
+uint64_t pool_bytes(uint64_t b);
+uint64_t pool_used_bytes();
 
 // Looks up (once per device; the device must be current) the device address
 // of the fixed scratch array of the allocation-free entry point.

@@ -150,6 +150,12 @@ hipError_t launch_eval(int kind, const EvalArgs& a, hipStream_t s);
 // start at out + set_part_base[s] + s
 hipError_t launch_batch_offsets(const uint32_t* global_offsets, const uint32_t* set_part_base, uint32_t sets,
                                 uint32_t total_entries, uint32_t* out, hipStream_t s);
+// batched alpha < 1 builds: where partition p's remap entries start in the
+// remap output, for p < np: set_remap_base[s] + goff[p] - goff[pb[s]] with
+// s the set owning p and goff the global remap offsets
+hipError_t launch_batch_remap_bases(const uint32_t* goff, const uint32_t* set_part_base,
+                                    const uint32_t* set_remap_base, uint32_t sets, uint32_t np, uint32_t* out,
+                                    hipStream_t s);
 hipError_t launch_check_bijection(const uint32_t* pos, uint64_t count, uint64_t n, uint32_t* bitmap,
                                   unsigned long long* bad, hipStream_t s);
 hipError_t launch_copy16(void* dst, const void* src, uint64_t words, hipStream_t s);

@@ -2,16 +2,29 @@
 // reference, which builds one MPHF per synchronous MPHFbuilder::build call,
 // mphf_builder.h:315-321).
 //
-// Requests are queued; a scheduler admits them in arrival order, letting a
-// later request overtake one that does not fit yet (backfill), whenever a
-// worker slot is idle and the request's device-memory estimate fits the HBM
-// budget.  Every slot owns a phobic_ctx (its own stream and scratch pool), so
-// admitted builds run concurrently on the device; results stay in device
-// memory until fetched or released.
+// Requests queue in arrival order.  Whenever a worker slot is idle the
+// scheduler forms a job from the oldest request that fits the HBM budget
+// (backfill: a later request may overtake one that does not fit yet).  A
+// small request takes every queued small request of its configuration with
+// it: the job is one flattened launch over all of their keys (a group, built
+// by phobic::group_build), whose per-request outputs equal building each
+// alone.  Large requests run alone.  Every slot owns a phobic_ctx (its own
+// stream and scratch), so jobs run concurrently on the device; results stay
+// in device memory until released.
+//
+// Memory accounting (all in the pool's 2 MiB units):
+//   kept      outputs of completed, unreleased requests (one block per job,
+//             freed with its last request)
+//   running   staging + outputs + scratch growth charged to running jobs
+//   retained  scratch each slot's context keeps between jobs
+// hbm_in_use = kept + running + retained; a job is admitted when it fits
+// the budget beside that (idle slots give their retained scratch back when
+// nothing else would ever fit).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -29,6 +42,19 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+constexpr uint64_t kGroupMaxKeys = 4000000;  // requests up to this size share launches
+constexpr uint64_t kGroupKeys = 64000000;    // keys of one shared launch at most
+constexpr uint32_t kGroupSets = 1024;        // requests of one shared launch at most
+
+uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+// Device memory holding the outputs of one job (one or several requests).
+struct Block {
+  void* ptr = nullptr;
+  uint64_t bytes = 0;  // pool units
+  uint32_t refs = 0;   // unreleased successful requests whose outputs live here
+};
+
 struct Request {
   uint64_t id = 0;
   phobic_request req{};
@@ -36,25 +62,59 @@ struct Request {
   std::vector<uint32_t> fulcs;  // copy of a custom table
   uint32_t np = 0, B = 0;
   uint64_t remap_cap = 0;
-  uint64_t est_bytes = 0;      // admission estimate (inputs + scratch + outputs)
-  uint64_t keep_bytes = 0;     // outputs kept on the device after completion
   Clock::time_point t_submit, t_start, t_done;
-  int state = 0;               // 0 queued, 1 running, 2 done
+  int state = 0;  // 0 queued, 1 running, 2 done
   int status = PHOBIC_OK;
-  // device state
-  void* d_in = nullptr;        // staged keys (host requests) or null
-  void* d_out = nullptr;       // pilots | offsets | remap | remap_offsets | status
+  std::shared_ptr<Block> block;
   phobic_outputs outs{};
-  uint32_t* d_status = nullptr;
+};
+
+// Output block of a single request: pilots | offsets | remap | remap offsets | status.
+uint64_t single_out_bytes(const Request& r) {
+  return align256((uint64_t)r.np * r.B * 4) + align256(((uint64_t)r.np + 1) * 4) + align256(r.remap_cap * 4) +
+         align256(((uint64_t)r.np + 1) * 4) + 256;
+}
+uint64_t single_in_bytes(const Request& r) {
+  if (r.req.keys_on_device) return 0;  // built in place
+  if (r.req.kind == PHOBIC_KEYS_BYTES) return align256(r.req.bytes_len) + (r.req.n + 1) * 8 + 256;
+  return r.req.n * 8;
+}
+
+// Output block of a group: pilots | offsets | remap | remap offsets | status
+// over the flattened build.
+uint64_t group_out_bytes(const phobic::GroupLayout& gl) {
+  const uint64_t entries = (uint64_t)gl.np + gl.sets;
+  const bool remap = gl.a_ppm < 1000000u;
+  return align256((uint64_t)gl.np * gl.B * 4) + align256(entries * 4) +
+         (remap ? align256((uint64_t)gl.remap_base[gl.sets] * 4) + align256(entries * 4) : 0) + 256;
+}
+
+struct Job {
+  std::vector<std::shared_ptr<Request>> reqs;
+  bool group = false;
+  phobic::GroupLayout gl;
+  uint64_t in_bytes = 0;  // group staging: keys (+ string offsets)
+  uint64_t charge = 0;    // counted in `running` while the job runs
 };
 
 struct Slot {
   phobic_ctx* ctx = nullptr;
-  std::shared_ptr<Request> job;
+  std::shared_ptr<Job> job;
   std::thread th;
+  uint64_t retained = 0;
 };
 
-uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+bool groupable(const Request& r) {
+  return r.req.n > 0 && r.req.n <= kGroupMaxKeys && !r.cfg.fulcrums &&
+         (r.req.kind == PHOBIC_KEYS_U64 || (r.req.kind == PHOBIC_KEYS_BYTES && !r.req.keys_on_device));
+}
+
+double alpha_of(const phobic_config& c) { return c.alpha == 0.0 ? 1.0 : c.alpha; }
+
+bool same_class(const Request& a, const Request& b) {
+  return a.req.kind == b.req.kind && a.cfg.lambda == b.cfg.lambda && a.cfg.partition_size == b.cfg.partition_size &&
+         a.cfg.buckets == b.cfg.buckets && a.cfg.seed == b.cfg.seed && alpha_of(a.cfg) == alpha_of(b.cfg);
+}
 
 }  // namespace
 
@@ -65,61 +125,150 @@ struct phobic_server {
   std::deque<std::shared_ptr<Request>> queue;
   std::map<uint64_t, std::shared_ptr<Request>> all;
   std::vector<Slot> slots;
-  hipStream_t free_stream = nullptr;  // releases of kept outputs
+  hipStream_t free_stream = nullptr;  // frees of released outputs
   uint64_t next_id = 1;
-  uint64_t mem_in_use = 0;
+  uint64_t kept = 0, running = 0;
   bool stop = false;
   // statistics
   std::vector<double> latencies_ms;
   uint64_t submitted = 0, completed = 0, rejected = 0, keys_built = 0;
+  uint64_t grouped_launches = 0, grouped_requests = 0;
   Clock::time_point t_first{}, t_last{};
   bool any = false;
 };
 
 namespace {
 
-// Device bytes a request may need: inputs, the build's scratch (about 56
-// bytes per key for partitioning plus the search slice) and outputs.
-void estimate(Request& r) {
-  const uint64_t n = r.req.n;
-  const uint64_t in = r.req.kind == PHOBIC_KEYS_BYTES ? (r.req.keys_on_device ? 0 : r.req.bytes_len + (n + 1) * 8)
-                                                      : (r.req.keys_on_device ? 0 : n * 8);
-  const uint64_t NB = (uint64_t)r.np * r.B;
-  r.keep_bytes = align256(NB * 4) + align256(((uint64_t)r.np + 1) * 4) + align256(r.remap_cap * 4) +
-                 align256(((uint64_t)r.np + 1) * 4) + 256;
-  const uint64_t cap = r.cfg.partition_size + r.cfg.partition_size / 2;
-  const uint64_t scratch = (uint64_t)(r.np + 128) * cap * 24 + (uint64_t)r.np * 16;
-  r.est_bytes = in + scratch + r.keep_bytes;
+uint64_t retained_total(const phobic_server* s) {
+  uint64_t t = 0;
+  for (const auto& sl : s->slots) t += sl.retained;
+  return t;
+}
+
+uint64_t in_use(const phobic_server* s) { return s->kept + s->running + retained_total(s); }
+
+// Charge of a job on a slot's context: staging, outputs, scratch growth.
+uint64_t job_charge(const Job& j, const phobic_ctx* ctx) {
+  const Request& r0 = *j.reqs[0];
+  if (j.group) {
+    return phobic::pool_bytes(j.in_bytes) + phobic::pool_bytes(group_out_bytes(j.gl)) +
+           phobic::group_scratch_growth(ctx, &r0.cfg, j.gl);
+  }
+  return phobic::pool_bytes(single_in_bytes(r0)) + phobic::pool_bytes(single_out_bytes(r0)) +
+         phobic::build_scratch_growth(ctx, &r0.cfg, r0.req.n);
+}
+
+// The job request `it` leads: itself, or a group with every queued small
+// request of its configuration (in queue order, within the group limits).
+// Caller holds s->mu.
+std::shared_ptr<Job> form_job(phobic_server* s, std::deque<std::shared_ptr<Request>>::iterator it) {
+  std::shared_ptr<Job> j;
+  try {
+    j = std::make_shared<Job>();
+    j->reqs.push_back(*it);
+    if (groupable(**it)) {
+      uint64_t keys = (*it)->req.n;
+      for (auto jt = std::next(it); jt != s->queue.end() && j->reqs.size() < kGroupSets; ++jt)
+        if (groupable(**jt) && same_class(**it, **jt) && keys + (*jt)->req.n <= kGroupKeys) {
+          j->reqs.push_back(*jt);
+          keys += (*jt)->req.n;
+        }
+    }
+    if (j->reqs.size() > 1) {
+      std::vector<uint64_t> sizes;
+      sizes.reserve(j->reqs.size());
+      uint64_t in = 0;
+      for (const auto& r : j->reqs) {
+        sizes.push_back(r->req.n);
+        in += r->req.kind == PHOBIC_KEYS_BYTES ? r->req.bytes_len : r->req.n * 8;
+      }
+      if (phobic::group_layout(&j->reqs[0]->cfg, sizes.data(), (uint32_t)sizes.size(), &j->gl) != PHOBIC_OK) {
+        j->reqs.resize(1);  // build it alone
+      } else {
+        j->group = true;
+        j->in_bytes = align256(in) + (j->reqs[0]->req.kind == PHOBIC_KEYS_BYTES ? (j->gl.n + 1) * 8 : 0) + 256;
+      }
+    }
+  } catch (...) {
+    return nullptr;
+  }
+  return j;
+}
+
+void start_job(phobic_server* s, Slot& sl, const std::shared_ptr<Job>& j) {
+  for (const auto& r : j->reqs) {
+    r->state = 1;
+    auto qt = std::find(s->queue.begin(), s->queue.end(), r);
+    if (qt != s->queue.end()) s->queue.erase(qt);
+  }
+  s->running += j->charge;
+  sl.job = j;
+  s->cv_work.notify_all();
 }
 
 void try_admit(phobic_server* s) {
   // caller holds s->mu
-  for (auto it = s->queue.begin(); it != s->queue.end();) {
-    Slot* idle = nullptr;
-    for (auto& sl : s->slots)
-      if (!sl.job) {
-        idle = &sl;
+  for (;;) {
+    bool idle = false, busy = false;
+    for (const auto& sl : s->slots) (sl.job ? busy : idle) = true;
+    if (!idle || s->queue.empty()) return;
+    bool started = false;
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
+      std::shared_ptr<Job> j = form_job(s, it);
+      if (!j) return;
+      // the idle slot whose retained scratch serves the job best
+      Slot* best = nullptr;
+      uint64_t best_charge = 0;
+      for (auto& sl : s->slots) {
+        if (sl.job) continue;
+        const uint64_t c = job_charge(*j, sl.ctx);
+        if (!best || c < best_charge) {
+          best = &sl;
+          best_charge = c;
+        }
+      }
+      // a group that does not fit is built as its leading request alone
+      if (j->group && in_use(s) + best_charge > s->cfg.hbm_budget) {
+        j->reqs.resize(1);
+        j->group = false;
+        best_charge = job_charge(*j, best->ctx);
+      }
+      if (in_use(s) + best_charge > s->cfg.hbm_budget && !busy && retained_total(s)) {
+        // nothing runs and retained scratch is in the way: the idle slots
+        // give it back
+        for (auto& sl : s->slots) {
+          phobic::ctx_trim(sl.ctx);
+          sl.retained = 0;
+        }
+        best_charge = job_charge(*j, best->ctx);
+        if (j->group && in_use(s) + best_charge > s->cfg.hbm_budget) {
+          j->reqs.resize(1);
+          j->group = false;
+          best_charge = job_charge(*j, best->ctx);
+        }
+      }
+      if (in_use(s) + best_charge <= s->cfg.hbm_budget) {
+        j->charge = best_charge;
+        start_job(s, *best, j);  // erases the job's requests from the queue
+        started = true;
         break;
       }
-    if (!idle) return;
-    auto& r = *it;
-    if (s->mem_in_use + r->est_bytes <= s->cfg.hbm_budget || s->mem_in_use == 0) {
-      s->mem_in_use += r->est_bytes;
-      r->state = 1;
-      idle->job = r;
-      it = s->queue.erase(it);
-      s->cv_work.notify_all();
-    } else {
-      ++it;  // backfill: a smaller request behind it may fit
     }
+    if (!started) return;
   }
 }
 
-int run_request(phobic_ctx* ctx, Request& r, hipStream_t stream) {
+void fail_outputs(Request& r) { r.outs = phobic_outputs{}; }
+
+// One request built alone.  Returns its status; on success r.block holds its
+// outputs (owned by the caller's accounting).
+int run_single(phobic_ctx* ctx, Request& r, hipStream_t stream) {
   const uint64_t n = r.req.n, NB = (uint64_t)r.np * r.B;
-  // outputs kept on the device until released
-  if (phobic::dev_alloc(&r.d_out, r.keep_bytes, stream) != hipSuccess) return PHOBIC_E_OOM;
-  uint8_t* p = static_cast<uint8_t*>(r.d_out);
+  auto block = std::make_shared<Block>();
+  block->bytes = phobic::pool_bytes(single_out_bytes(r));
+  if (phobic::dev_alloc(&block->ptr, single_out_bytes(r), stream) != hipSuccess) return PHOBIC_E_OOM;
+  r.block = block;
+  uint8_t* p = static_cast<uint8_t*>(block->ptr);
   r.outs.pilots = reinterpret_cast<uint32_t*>(p);
   p += align256(NB * 4);
   r.outs.offsets = reinterpret_cast<uint32_t*>(p);
@@ -128,74 +277,180 @@ int run_request(phobic_ctx* ctx, Request& r, hipStream_t stream) {
   p += align256(r.remap_cap * 4);
   r.outs.remap_offsets = r.remap_cap ? reinterpret_cast<uint32_t*>(p) : nullptr;
   p += align256(((uint64_t)r.np + 1) * 4);
-  r.d_status = reinterpret_cast<uint32_t*>(p);
+  uint32_t* d_status = reinterpret_cast<uint32_t*>(p);
   const void* keys = r.req.keys;
   const uint64_t* offs = r.req.offsets;
+  void* staging = nullptr;
+  auto done = [&](int st) {
+    if (staging) (void)phobic::dev_free(staging, stream);
+    return st;
+  };
   if (!r.req.keys_on_device) {  // stage host keys
     const uint64_t kb = r.req.kind == PHOBIC_KEYS_BYTES ? r.req.bytes_len : n * 8;
     const uint64_t ob = r.req.kind == PHOBIC_KEYS_BYTES ? (n + 1) * 8 : 0;
-    if (phobic::dev_alloc(&r.d_in, align256(kb) + ob + 256, stream) != hipSuccess) return PHOBIC_E_OOM;
-    uint8_t* q = static_cast<uint8_t*>(r.d_in);
-    if (kb && hipMemcpyAsync(q, r.req.keys, kb, hipMemcpyHostToDevice, stream) != hipSuccess) return PHOBIC_E_HIP;
+    if (phobic::dev_alloc(&staging, single_in_bytes(r), stream) != hipSuccess) return done(PHOBIC_E_OOM);
+    uint8_t* q = static_cast<uint8_t*>(staging);
+    if (kb && hipMemcpyAsync(q, r.req.keys, kb, hipMemcpyHostToDevice, stream) != hipSuccess) return done(PHOBIC_E_HIP);
     if (ob && hipMemcpyAsync(q + align256(kb), r.req.offsets, ob, hipMemcpyHostToDevice, stream) != hipSuccess)
-      return PHOBIC_E_HIP;
+      return done(PHOBIC_E_HIP);
     keys = q;
     offs = reinterpret_cast<const uint64_t*>(q + align256(kb));
   }
   int st;
   if (r.req.kind == PHOBIC_KEYS_BYTES)
-    st = phobic_build_bytes_device(ctx, static_cast<const uint8_t*>(keys), offs, n, &r.cfg, &r.outs, r.d_status,
+    st = phobic_build_bytes_device(ctx, static_cast<const uint8_t*>(keys), offs, n, &r.cfg, &r.outs, d_status,
                                    stream);
   else
-    st = phobic_build_u64_device(ctx, static_cast<const uint64_t*>(keys), n, &r.cfg, &r.outs, r.d_status, stream);
-  if (st) return st;
+    st = phobic_build_u64_device(ctx, static_cast<const uint64_t*>(keys), n, &r.cfg, &r.outs, d_status, stream);
+  if (st) return done(st);
   uint32_t bits = 0;
-  if (hipMemcpyAsync(&bits, r.d_status, 4, hipMemcpyDeviceToHost, stream) != hipSuccess) return PHOBIC_E_HIP;
-  if (hipStreamSynchronize(stream) != hipSuccess) return PHOBIC_E_HIP;
-  if (r.d_in) {
-    (void)phobic::dev_free(r.d_in, stream);
-    r.d_in = nullptr;
-  }
+  if (hipMemcpyAsync(&bits, d_status, 4, hipMemcpyDeviceToHost, stream) != hipSuccess) return done(PHOBIC_E_HIP);
+  if (hipStreamSynchronize(stream) != hipSuccess) return done(PHOBIC_E_HIP);
   for (int c = 1; c < 32; c++)
-    if (bits & (1u << c)) return c;
+    if (bits & (1u << c)) return done(c);
+  return done(PHOBIC_OK);
+}
+
+// A group built as one launch.  On success every request's outputs point
+// into the shared block.
+int run_group(phobic_ctx* ctx, Job& j, hipStream_t stream) {
+  const phobic::GroupLayout& gl = j.gl;
+  const Request& r0 = *j.reqs[0];
+  const bool bytes = r0.req.kind == PHOBIC_KEYS_BYTES;
+  const bool remap = gl.a_ppm < 1000000u;
+  auto block = std::make_shared<Block>();
+  const uint64_t ob = group_out_bytes(gl);
+  block->bytes = phobic::pool_bytes(ob);
+  if (phobic::dev_alloc(&block->ptr, ob, stream) != hipSuccess) return PHOBIC_E_OOM;
+  void* staging = nullptr;
+  auto fail = [&](int st) {
+    if (staging) (void)phobic::dev_free(staging, stream);
+    (void)phobic::dev_free(block->ptr, stream);
+    block->ptr = nullptr;
+    return st;
+  };
+  if (phobic::dev_alloc(&staging, j.in_bytes, stream) != hipSuccess) return fail(PHOBIC_E_OOM);
+  uint8_t* q = static_cast<uint8_t*>(staging);
+  uint64_t key_bytes = 0;
+  for (const auto& r : j.reqs) key_bytes += bytes ? r->req.bytes_len : r->req.n * 8;
+  uint64_t* d_offs = reinterpret_cast<uint64_t*>(q + align256(key_bytes));
+  std::vector<uint64_t> offs;
+  try {
+    if (bytes) offs.resize(gl.n + 1);
+  } catch (...) {
+    return fail(PHOBIC_E_OOM);
+  }
+  uint64_t at = 0, key = 0;
+  for (const auto& r : j.reqs) {
+    const uint64_t kb = bytes ? r->req.offsets[r->req.n] - r->req.offsets[0] : r->req.n * 8;
+    const void* src = bytes ? static_cast<const uint8_t*>(r->req.keys) + r->req.offsets[0] : r->req.keys;
+    const hipMemcpyKind dir = r->req.keys_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (kb && hipMemcpyAsync(q + at, src, kb, dir, stream) != hipSuccess) return fail(PHOBIC_E_HIP);
+    if (bytes)
+      for (uint64_t i = 0; i < r->req.n; i++) offs[key + i] = at + (r->req.offsets[i] - r->req.offsets[0]);
+    at += kb;
+    key += r->req.n;
+  }
+  if (bytes) {
+    offs[gl.n] = at;
+    if (hipMemcpyAsync(d_offs, offs.data(), (gl.n + 1) * 8, hipMemcpyHostToDevice, stream) != hipSuccess)
+      return fail(PHOBIC_E_HIP);
+  }
+  uint8_t* p = static_cast<uint8_t*>(block->ptr);
+  const uint64_t entries = (uint64_t)gl.np + gl.sets;
+  phobic_outputs bo{};
+  bo.pilots = reinterpret_cast<uint32_t*>(p);
+  p += align256((uint64_t)gl.np * gl.B * 4);
+  bo.offsets = reinterpret_cast<uint32_t*>(p);
+  p += align256(entries * 4);
+  if (remap) {
+    bo.remap = reinterpret_cast<uint32_t*>(p);
+    p += align256((uint64_t)gl.remap_base[gl.sets] * 4);
+    bo.remap_offsets = reinterpret_cast<uint32_t*>(p);
+    p += align256(entries * 4);
+  }
+  uint32_t* d_status = reinterpret_cast<uint32_t*>(p);
+  int st = phobic::group_build(ctx, &r0.cfg, bytes ? 1 : 0, q, bytes ? d_offs : nullptr, gl, bo, d_status, stream);
+  if (st) return fail(st);
+  uint32_t bits = 0;
+  if (hipMemcpyAsync(&bits, d_status, 4, hipMemcpyDeviceToHost, stream) != hipSuccess) return fail(PHOBIC_E_HIP);
+  if (hipStreamSynchronize(stream) != hipSuccess) return fail(PHOBIC_E_HIP);
+  for (int c = 1; c < 32; c++)
+    if (bits & (1u << c)) return fail(c);
+  (void)phobic::dev_free(staging, stream);
+  for (uint32_t s = 0; s < gl.sets; s++) {
+    Request& r = *j.reqs[s];
+    r.block = block;
+    r.outs.pilots = bo.pilots + (uint64_t)gl.B * gl.part_base[s];
+    r.outs.offsets = bo.offsets + gl.part_base[s] + s;
+    r.outs.remap = remap ? bo.remap + gl.remap_base[s] : nullptr;
+    r.outs.remap_offsets = remap ? bo.remap_offsets + gl.part_base[s] + s : nullptr;
+  }
   return PHOBIC_OK;
 }
 
 void worker(phobic_server* s, size_t idx, hipStream_t stream) {
   (void)hipSetDevice(s->cfg.device);
+  phobic_ctx* ctx = s->slots[idx].ctx;
   for (;;) {
-    std::shared_ptr<Request> job;
+    std::shared_ptr<Job> job;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stop || s->slots[idx].job; });
       if (s->stop && !s->slots[idx].job) return;
       job = s->slots[idx].job;
-      job->t_start = Clock::now();
+      const auto now = Clock::now();
+      for (const auto& r : job->reqs) r->t_start = now;
     }
-    const int st = run_request(s->slots[idx].ctx, *job, stream);
+    std::vector<int> st(job->reqs.size(), PHOBIC_OK);
+    bool grouped = false;
+    if (job->group && run_group(ctx, *job, stream) == PHOBIC_OK) {
+      grouped = true;
+    } else {
+      // alone, or every request of a group that failed (each then reports
+      // its own status)
+      for (size_t i = 0; i < job->reqs.size(); i++) {
+        st[i] = run_single(ctx, *job->reqs[i], stream);
+        if (st[i] != PHOBIC_OK) {
+          if (job->reqs[i]->block && job->reqs[i]->block->ptr) (void)phobic::dev_free(job->reqs[i]->block->ptr, stream);
+          job->reqs[i]->block.reset();
+          fail_outputs(*job->reqs[i]);
+        }
+      }
+    }
+    (void)hipStreamSynchronize(stream);
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      job->status = st;
-      job->t_done = Clock::now();
-      job->state = 2;
+      const auto now = Clock::now();
+      s->running -= job->charge;
+      s->slots[idx].retained = phobic::ctx_scratch_bytes(ctx);
       s->slots[idx].job.reset();
-      // scratch and staging returned; outputs stay until released
-      s->mem_in_use -= job->est_bytes - (st == PHOBIC_OK ? job->keep_bytes : 0);
-      if (st != PHOBIC_OK && job->d_out) {
-        (void)phobic::dev_free(job->d_out, stream);
-        job->d_out = nullptr;
+      std::vector<Block*> counted;
+      for (size_t i = 0; i < job->reqs.size(); i++) {
+        Request& r = *job->reqs[i];
+        r.status = st[i];
+        r.t_done = now;
+        r.state = 2;
+        if (st[i] == PHOBIC_OK) {
+          Block* b = r.block.get();
+          if (std::find(counted.begin(), counted.end(), b) == counted.end()) {
+            s->kept += b->bytes;
+            counted.push_back(b);
+          }
+          b->refs++;
+          s->keys_built += r.req.n;
+        }
+        s->completed++;
+        try {  // a sample that cannot be stored is dropped from the statistics
+          s->latencies_ms.push_back(std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count());
+        } catch (...) {
+        }
       }
-      if (job->d_in) {  // staging left behind by a failed request
-        (void)phobic::dev_free(job->d_in, stream);
-        job->d_in = nullptr;
+      if (grouped) {
+        s->grouped_launches++;
+        s->grouped_requests += job->reqs.size();
       }
-      s->completed++;
-      if (st == PHOBIC_OK) s->keys_built += job->req.n;
-      try {  // a sample that cannot be stored is dropped from the statistics
-        s->latencies_ms.push_back(std::chrono::duration<double, std::milli>(job->t_done - job->t_submit).count());
-      } catch (...) {
-      }
-      s->t_last = job->t_done;
+      s->t_last = now;
       try_admit(s);
     }
     s->cv_done.notify_all();
@@ -259,7 +514,14 @@ int phobic_server_submit(phobic_server* s, const phobic_request* req, uint64_t* 
   if (!s || !req || !id) return PHOBIC_E_INVALID_CONFIG;
   if (req->kind != PHOBIC_KEYS_U64 && req->kind != PHOBIC_KEYS_BYTES) return PHOBIC_E_INVALID_CONFIG;
   if (req->n && !req->keys) return PHOBIC_E_INVALID_CONFIG;
-  if (req->kind == PHOBIC_KEYS_BYTES && !req->offsets) return PHOBIC_E_INVALID_CONFIG;
+  if (req->kind == PHOBIC_KEYS_BYTES) {
+    if (!req->offsets) return PHOBIC_E_INVALID_CONFIG;
+    if (!req->keys_on_device) {  // the kernels read bytes [offsets[0], offsets[n])
+      for (uint64_t i = 0; i < req->n; i++)
+        if (req->offsets[i + 1] < req->offsets[i]) return PHOBIC_E_INVALID_CONFIG;
+      if (req->offsets[req->n] > req->bytes_len) return PHOBIC_E_INVALID_CONFIG;
+    }
+  }
   std::shared_ptr<Request> r;
   try {
     r = std::make_shared<Request>();
@@ -276,9 +538,11 @@ int phobic_server_submit(phobic_server* s, const phobic_request* req, uint64_t* 
   if (st) return st;
   st = phobic_remap_capacity(&r->cfg, req->n, &r->remap_cap);
   if (st) return st;
-  estimate(*r);
+  // what it needs alone on a slot that holds no scratch yet
+  const uint64_t need = phobic::pool_bytes(single_in_bytes(*r)) + phobic::pool_bytes(single_out_bytes(*r)) +
+                        phobic::build_scratch_growth(s->slots[0].ctx, &r->cfg, req->n, true);
   std::lock_guard<std::mutex> lk(s->mu);
-  if (r->est_bytes > s->cfg.hbm_budget) {
+  if (need > s->cfg.hbm_budget) {
     s->rejected++;
     return PHOBIC_E_OOM;
   }
@@ -336,14 +600,15 @@ int phobic_server_result(phobic_server* s, uint64_t id, uint32_t* np, uint32_t* 
   {
     std::lock_guard<std::mutex> lk(s->mu);
     r = find(s, id);
+    if (!r) return PHOBIC_E_NOT_FOUND;
+    if (r->state != 2) return PHOBIC_E_BUSY;
   }
-  if (!r) return PHOBIC_E_NOT_FOUND;
-  if (r->state != 2) return PHOBIC_E_BUSY;
   if (r->status != PHOBIC_OK) return r->status;
   if (np) *np = r->np;
   if (buckets) *buckets = r->B;
   if (remap_entries) {
     uint32_t used = 0;
+    (void)hipSetDevice(s->cfg.device);
     if (r->remap_cap && hipMemcpy(&used, r->outs.remap_offsets + r->np, 4, hipMemcpyDeviceToHost) != hipSuccess)
       return PHOBIC_E_HIP;
     *remap_entries = used;
@@ -377,20 +642,27 @@ int phobic_server_fetch(phobic_server* s, uint64_t id, const phobic_outputs* hos
 
 int phobic_server_release(phobic_server* s, uint64_t id) {
   if (!s) return PHOBIC_E_INVALID_CONFIG;
-  std::shared_ptr<Request> r;
+  std::shared_ptr<Block> last;  // the block this release empties
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    r = find(s, id);
+    auto r = find(s, id);
     if (!r) return PHOBIC_E_NOT_FOUND;
     if (r->state != 2) return PHOBIC_E_BUSY;
     s->all.erase(id);
-    if (r->d_out) s->mem_in_use -= r->keep_bytes;
+    if (r->status == PHOBIC_OK && r->block && --r->block->refs == 0) last = r->block;
+    r->block.reset();
   }
-  (void)hipSetDevice(s->cfg.device);
-  if (r->d_out) (void)phobic::dev_free(r->d_out, s->free_stream);
-  r->d_out = nullptr;
+  if (last) {
+    // freed before it leaves the count, so the count never falls below
+    // what the pool holds
+    (void)hipSetDevice(s->cfg.device);
+    (void)phobic::dev_free(last->ptr, s->free_stream);
+    (void)hipStreamSynchronize(s->free_stream);
+    last->ptr = nullptr;
+  }
   {
     std::lock_guard<std::mutex> lk(s->mu);
+    if (last) s->kept -= last->bytes;
     try_admit(s);
   }
   return PHOBIC_OK;
@@ -404,7 +676,11 @@ int phobic_server_stats_get(phobic_server* s, phobic_server_stats* out) {
   out->completed = s->completed;
   out->rejected = s->rejected;
   out->keys_built = s->keys_built;
-  out->hbm_in_use = s->mem_in_use;
+  out->hbm_in_use = in_use(s);
+  out->grouped_launches = s->grouped_launches;
+  out->grouped_requests = s->grouped_requests;
+  (void)hipSetDevice(s->cfg.device);
+  out->pool_used = phobic::pool_used_bytes();
   std::vector<double> v;
   try {
     v = s->latencies_ms;
@@ -438,9 +714,14 @@ int phobic_server_destroy(phobic_server* s) {
   for (auto& sl : s->slots)
     if (sl.th.joinable()) sl.th.join();
   (void)hipSetDevice(s->cfg.device);
+  std::vector<Block*> freed;
   for (auto& kv : s->all) {
-    if (kv.second->d_out) (void)phobic::dev_free(kv.second->d_out, s->free_stream);
-    if (kv.second->d_in) (void)phobic::dev_free(kv.second->d_in, s->free_stream);
+    Block* b = kv.second->block.get();
+    if (b && b->ptr && std::find(freed.begin(), freed.end(), b) == freed.end()) {
+      (void)phobic::dev_free(b->ptr, s->free_stream);
+      b->ptr = nullptr;
+      freed.push_back(b);
+    }
   }
   (void)hipStreamSynchronize(s->free_stream);
   (void)hipStreamDestroy(s->free_stream);

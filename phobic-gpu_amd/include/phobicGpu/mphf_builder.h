@@ -66,6 +66,7 @@ class MPHFbuilder {
   template <typename Mphf, typename keyType>
   HostTimer build(const std::vector<keyType>& keys, Mphf& f) {
     HostTimer timer;
+    timer.addLabel("initial_hash");  // on the device, inside the build (below)
     if (!ctx) check(phobic_ctx_create(device, &ctx), "phobic_ctx_create");
     timer.addLabel("allocation");
     const phobic_config cfg = abiConfig();
@@ -91,7 +92,6 @@ class MPHFbuilder {
       std::vector<uint8_t> bytes(offs[n]);
       for (uint64_t i = 0; i < n; i++)
         if (!keys[i].empty()) std::memcpy(bytes.data() + offs[i], keys[i].data(), keys[i].size());
-      timer.addLabel("input_packing");
       st = phobic_build_bytes(ctx, bytes.data(), offs.data(), n, &cfg, &out, &t);
     } else if constexpr (std::is_same_v<keyType, std::pair<uint64_t, uint64_t>> || std::is_same_v<keyType, Key>) {
       // 16 bytes per key, hashed as the reference hashes a pair or a Key
@@ -113,12 +113,33 @@ class MPHFbuilder {
       static_assert(sizeof(keyType) == 0, "supported key types: uint64_t, std::string, std::pair<u64,u64>, Key");
     }
     check(st, "MPHFbuilder::build");
-    const double gpu0 = timer.elapsed() - t.total_ms * 1e6;
-    double acc = gpu0;
-    for (uint32_t i = 0; i < t.count; i++) {
-      acc += t.ms[i] * 1e6;
-      timer.addLabelManually(std::string("GPU_") + t.names[i], acc);
-    }
+    // The reference's labels (mphf_builder.h:138-146, 229-290), each the
+    // end of its split: keys are hashed on the device (initial_hash is
+    // empty: it marks the start), kernels are enqueued inside the build call (setup_commands is
+    // empty), bucket sizes are counted by the routing pass that also hashes
+    // and maps every key, bucket sorting and the in-partition
+    // redistribution are fused into the search, partition offsets need no
+    // separate application, and results are copied, not mapped.
+    auto stage = [&](const char* name) {
+      for (uint32_t i = 0; i < t.count; i++)
+        if (std::strcmp(t.names[i], name) == 0) return (double)t.ms[i] * 1e6;
+      return 0.0;
+    };
+    // input_transfer: host-side packing of the keys and their copy in
+    double acc = timer.elapsed() - (t.total_ms + t.result_transfer_ms) * 1e6;
+    timer.addLabelManually("input_transfer", acc);
+    timer.addLabelManually("setup_commands", acc);
+    acc += stage("hash_route");
+    timer.addLabelManually("GPU_bucket_sizes", acc);
+    timer.addLabelManually("GPU_bucket_sorting", acc);
+    acc += stage("partition_offsets");
+    timer.addLabelManually("GPU_partition_offsets", acc);
+    timer.addLabelManually("GPU_apply_partition_offsets", acc);
+    acc += stage("partition_split");
+    timer.addLabelManually("GPU_key_redistribution", acc);
+    acc += stage("partition_search");
+    timer.addLabelManually("GPU_search", acc);
+    timer.addLabelManually("GPU_memory_map", acc);
     timer.addLabel("result_transfer");
     if (rcap) {
       remap.resize(remapOffsets[np]);

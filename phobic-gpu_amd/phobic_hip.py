@@ -85,11 +85,16 @@ class StageTimes(ctypes.Structure):
         ("ms", ctypes.c_float * MAX_STAGES),
         ("names", (ctypes.c_char * 32) * MAX_STAGES),
         ("total_ms", ctypes.c_float),
+        ("input_transfer_ms", ctypes.c_float),
+        ("result_transfer_ms", ctypes.c_float),
     ]
 
     def as_dict(self) -> dict:
         d = {self.names[i].value.decode(): float(self.ms[i]) for i in range(self.count)}
         d["total"] = float(self.total_ms)
+        if self.input_transfer_ms or self.result_transfer_ms:
+            d["input_transfer"] = float(self.input_transfer_ms)
+            d["result_transfer"] = float(self.result_transfer_ms)
         return d
 
 
@@ -463,6 +468,9 @@ class ServerStats(ctypes.Structure):
         ("mean_ms", ctypes.c_double),
         ("max_ms", ctypes.c_double),
         ("span_s", ctypes.c_double),
+        ("grouped_launches", ctypes.c_uint64),
+        ("grouped_requests", ctypes.c_uint64),
+        ("pool_used", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -499,7 +507,7 @@ class Server:
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         buf = data if data.size else np.zeros(1, np.uint8)
         n = offsets.size - 1
-        req = Request(KEYS_BYTES, 0, buf.ctypes.data, offsets.ctypes.data, int(offsets[-1]), n, cfg)
+        req = Request(KEYS_BYTES, 0, buf.ctypes.data, offsets.ctypes.data, int(data.size), n, cfg)
         return self._submit(req, (buf, offsets), cfg)
 
     def _submit(self, req: Request, hold: tuple, cfg: Config) -> int:

@@ -24,6 +24,17 @@ def test_example_builds_a_bijection(kind):
     assert "valid for 300000" in r.stdout
     bpk = float(r.stdout.split("total_bits=")[1].split()[0])
     assert 1.5 < bpk < 4.0
+    # the reference's timer labels in its order (mphf_builder.h:138-146,
+    # 229-290; printed by benchmark.cpp:81-90 as RESULT columns)
+    result = next(line for line in r.stdout.splitlines() if line.startswith("RESULT "))
+    cols = dict(tok.split("=", 1) for tok in result.split()[1:] if "=" in tok)
+    labels = ["initial_hash", "allocation", "input_transfer", "setup_commands", "GPU_bucket_sizes",
+              "GPU_bucket_sorting", "GPU_partition_offsets", "GPU_apply_partition_offsets", "GPU_key_redistribution",
+              "GPU_search", "GPU_memory_map", "result_transfer", "encoding"]
+    order = [tok.split("=", 1)[0] for tok in result.split()[1:] if tok.split("=", 1)[0] in labels]
+    assert order == labels
+    assert all(float(cols[k]) >= 0 for k in labels)
+    assert float(cols["GPU_search"]) > 0 and float(cols["GPU_bucket_sizes"]) > 0
 
 
 def test_every_key_type_builds_a_bijection():

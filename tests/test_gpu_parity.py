@@ -407,19 +407,14 @@ def gen_strings_ref(seed, offs):
         state = seed ^ ((i * 0xD1B54A32D192ED03) & M)
         r = 0
         for j in range(ln):
-            if j < 8:
-                d = (i >> (4 * j)) & 15
-                c = 48 + d if d < 10 else 97 + d - 10
-            else:
-                if (j - 8) % 8 == 0:
-                    state = (state + 0x9E3779B97F4A7C15) & M
-                    z = state
-                    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
-                    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
-                    r = z ^ (z >> 31)
-                c = 0x20 + (r & 0xFF) % 95
-                r >>= 8
-            out[a + j] = c
+            if j % 8 == 0:
+                state = (state + 0x9E3779B97F4A7C15) & M
+                z = state
+                z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+                z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+                r = z ^ (z >> 31)
+            out[a + j] = 0x20 + (r & 0xFF) % 95
+            r >>= 8
     return out
 
 

@@ -1,6 +1,9 @@
-"""Serving: requests submitted concurrently while others run produce the same
-results as building each one alone; admission respects the HBM budget."""
+"""Serving: requests submitted while others run give the oracle's results,
+alone or built together with other queued requests of their configuration in
+one launch; admission keeps the server's counted memory within the budget
+and never below what the device pool actually holds."""
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -11,66 +14,138 @@ import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 
-def request_mix(count, seed):
+def request_mix(count, seed, lo=1e4, hi=2e6):
+    """u64 (70 %) and printable string (30 %) requests, lambda {6,7,8},
+    alpha {0.99, 1}, sizes log-uniform in [lo, hi] (strings: an eighth)."""
     rng = np.random.default_rng(seed)
     reqs = []
     for i in range(count):
-        n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         lam = float(rng.choice([6.0, 7.0, 8.0]))
         alpha = float(rng.choice([0.99, 1.0]))
         cfg = ph.make_config(lam, 2048, 0, alpha)
         if rng.random() < 0.7:
-            reqs.append(("u64", po.splitmix64(5000 + i, n), cfg))
+            reqs.append(("u64", po.splitmix64(5000 + i, n), cfg, lam, alpha))
         else:
             m = max(1000, n // 8)
-            lens = rng.integers(8, 65, m)
             offs = np.zeros(m + 1, np.uint64)
-            offs[1:] = np.cumsum(lens)
+            offs[1:] = np.cumsum(rng.integers(8, 65, m))
             data = rng.integers(0x20, 0x7F, int(offs[-1]), dtype=np.uint8)
-            # make strings distinct: overwrite 8 bytes with the index (printable hex)
-            for j in range(m):
-                data[int(offs[j]):int(offs[j]) + 8] = np.frombuffer(b"%08x" % j, np.uint8)
-            reqs.append(("bytes", (data, offs), cfg))
+            reqs.append(("bytes", (data, offs), cfg, lam, alpha))
     return reqs
 
 
-def alone(ctx, kind, payload, cfg):
-    if kind == "u64":
-        return ctx.build_u64(payload, cfg)
-    return ctx.build_bytes(payload[0], payload[1], cfg)
+def submit(srv, req):
+    kind, payload, cfg = req[:3]
+    return srv.submit_u64(payload, cfg) if kind == "u64" else srv.submit_bytes(payload[0], payload[1], cfg)
 
 
-def same(a, b):
-    ok = np.array_equal(a.pilots, b.pilots) and np.array_equal(a.offsets, b.offsets)
-    if a.remap is not None or b.remap is not None:
-        ok = ok and np.array_equal(a.remap, b.remap) and np.array_equal(a.remap_offsets, b.remap_offsets)
-    return ok
+def check_against_oracle(r, req):
+    """Pilots, offsets and (alpha < 1) the remap table equal the oracle's."""
+    kind, payload, _, lam, alpha = req
+    hk = po.hash_u64(payload, 0) if kind == "u64" else po.hash_bytes(payload[0], payload[1], 0)
+    if alpha < 1.0:
+        st, pil, off, rm, roff = po.build_keys_alpha(hk, lam, 2048, alpha)
+        assert st == 0
+        assert np.array_equal(r.remap, rm) and np.array_equal(r.remap_offsets, roff)
+    else:
+        st, pil, off = po.build_keys(hk, lam, 2048)
+        assert st == 0
+    assert np.array_equal(r.pilots, pil) and np.array_equal(r.offsets, off)
 
 
-def test_results_alone_equal_in_the_mix(ctx):
+def test_results_alone_equal_in_the_mix():
     reqs = request_mix(24, 1)
     srv = ph.Server(slots=4, hbm_budget=64 << 30)
     ids = [None] * len(reqs)
 
-    def submit(lo, hi):  # two submitting threads, arrivals interleave with running builds
+    def go(lo, hi):  # two submitting threads: arrivals interleave with running launches
         for i in range(lo, hi):
-            kind, payload, cfg = reqs[i]
-            ids[i] = srv.submit_u64(payload, cfg) if kind == "u64" else srv.submit_bytes(payload[0], payload[1], cfg)
+            ids[i] = submit(srv, reqs[i])
 
-    th = [threading.Thread(target=submit, args=(0, 12)), threading.Thread(target=submit, args=(12, 24))]
+    th = [threading.Thread(target=go, args=(0, 12)), threading.Thread(target=go, args=(12, 24))]
     for t in th:
         t.start()
     for t in th:
         t.join()
-    for i, (kind, payload, cfg) in enumerate(reqs):
+    for i, req in enumerate(reqs):
         st, lat = srv.wait(ids[i])
         assert st == 0 and lat > 0
-        mix = srv.fetch(ids[i])
-        assert same(mix, alone(ctx, kind, payload, cfg)), i
+        check_against_oracle(srv.fetch(ids[i]), req)
         srv.release(ids[i])
     s = srv.stats()
-    assert s["completed"] == len(reqs) and s["rejected"] == 0 and s["hbm_in_use"] == 0
+    assert s["completed"] == len(reqs) and s["rejected"] == 0
     assert s["p99_ms"] >= s["p50_ms"] > 0
+    srv.close()
+
+
+def test_queued_small_requests_share_launches():
+    """One slot, busy with a large build while 60 small requests of three
+    configurations queue behind it: they are built together, in launches of
+    several requests each, and every result is the oracle's."""
+    srv = ph.Server(slots=1, hbm_budget=64 << 30)
+    big = srv.submit_u64(po.splitmix64(77, 30_000_000), ph.make_config(7.0, 2048))
+    rng = np.random.default_rng(3)
+    reqs = []
+    for i in range(60):
+        n = int(rng.integers(5_000, 150_000))
+        kind = ["u64", "u64", "bytes"][i % 3]
+        lam, alpha = [(7.0, 1.0), (7.5, 0.99), (6.0, 0.99)][i % 3]
+        cfg = ph.make_config(lam, 2048, 0, alpha)
+        if kind == "u64":
+            reqs.append(("u64", po.splitmix64(9000 + i, n), cfg, lam, alpha))
+        else:
+            m = n // 4
+            offs = np.zeros(m + 1, np.uint64)
+            offs[1:] = np.cumsum(rng.integers(8, 65, m))
+            data = rng.integers(0x20, 0x7F, int(offs[-1]), dtype=np.uint8)
+            reqs.append(("bytes", (data, offs), cfg, lam, alpha))
+    ids = [submit(srv, r) for r in reqs]
+    assert srv.wait(big)[0] == 0
+    for rid, req in zip(ids, reqs):
+        assert srv.wait(rid)[0] == 0
+        check_against_oracle(srv.fetch(rid), req)
+    s = srv.stats()
+    assert s["grouped_launches"] >= 3 and s["grouped_requests"] >= 30
+    for rid in ids + [big]:
+        srv.release(rid)
+    srv.close()
+
+
+def test_counted_memory_covers_the_pool_during_a_trace():
+    """Sampled while a mixed trace runs: the server's counted device memory is
+    never below what the library's pool holds beyond its level before the
+    server existed, and never above the budget."""
+    budget = 12 << 30
+    reqs = request_mix(40, 2, 1e4, 8e6)
+    srv = ph.Server(slots=4, hbm_budget=budget)
+    base = srv.stats()["pool_used"]
+    samples, stop = [], threading.Event()
+
+    def sample():
+        while not stop.is_set():
+            s = srv.stats()
+            samples.append((s["hbm_in_use"], s["pool_used"]))
+            time.sleep(0.0005)
+
+    t = threading.Thread(target=sample)
+    t.start()
+    try:
+        ids = [submit(srv, r) for r in reqs]
+        for i, rid in enumerate(ids):
+            assert srv.wait(rid)[0] == 0
+            if i % 2:
+                srv.release(rid)
+        for i, rid in enumerate(ids):
+            if i % 2 == 0:
+                srv.release(rid)
+    finally:
+        stop.set()
+        t.join()
+    assert len(samples) > 20
+    for counted, used in samples:
+        assert counted >= used - base, (counted, used, base)
+        assert counted <= budget
     srv.close()
 
 
@@ -84,6 +159,24 @@ def test_budget_admission_and_rejection():
         srv.submit_u64(po.splitmix64(8, 20_000_000), ph.make_config(7.0, 2048))
     assert e.value.status == 3
     assert srv.stats()["rejected"] == 1
+    srv.close()
+
+
+def test_malformed_string_requests_refused_at_submit():
+    """Host string requests whose offsets decrease, or end past the bytes
+    given, are refused before anything reaches the device."""
+    srv = ph.Server(slots=1, hbm_budget=1 << 30)
+    cfg = ph.make_config(6.0, 2048)
+    data = np.frombuffer(b"abcdefghijklmnopqrstuvwxyz012345", np.uint8)
+    with pytest.raises(ph.PhobicError) as e:
+        srv.submit_bytes(data, np.array([0, 8, 4, 16], np.uint64), cfg)
+    assert e.value.status == 1
+    with pytest.raises(ph.PhobicError) as e:
+        srv.submit_bytes(data, np.array([0, 8, 16, 40], np.uint64), cfg)
+    assert e.value.status == 1
+    rid = srv.submit_bytes(data, np.array([0, 8, 16, 32], np.uint64), cfg)
+    assert srv.wait(rid)[0] == 0
+    srv.release(rid)
     srv.close()
 
 
