@@ -203,42 +203,64 @@ def _mix_np(key, pilot):
     return (key >> np.uint32(16)) ^ key
 
 
-@pytest.mark.parametrize("n,lam,P,seed", [(3000, 4.0, 100, 21), (2500, 3.0, 64, 22), (6000, 7.0, 256, 23)])
-def test_search_minimal_pilots_bruteforce(n, lam, P, seed):
-    """Replays every partition: each stored V is valid and no smaller value is."""
+@pytest.mark.parametrize("n,lam,P,seed,alpha", [
+    (3000, 4.0, 100, 21, 1.0), (2500, 3.0, 64, 22, 1.0), (6000, 7.0, 256, 23, 1.0),
+    (6000, 7.0, 600, 24, 0.9),     # tables of m = ceil(ps / 0.9) slots, above 512
+    (4096, 7.5, 2048, 25, 0.99),   # the 1e9-key configuration: m crosses 2048
+    (5000, 6.0, 700, 26, 0.7),
+])
+def test_search_minimal_pilots_bruteforce(n, lam, P, seed, alpha):
+    """Replays every partition: each stored V is valid and no smaller value
+    is, over a table of m slots (m = ps at alpha 1, ceil(ps / alpha) below);
+    for alpha < 1 the remap table sends the k-th used slot of [ps, m) to the
+    k-th free slot of [0, ps)."""
     hk = _keys(n, seed=seed)
-    st, pil, off, bs, order = po.build_keys(hk, lam, P, intermediates=True)
+    _, _, _, bs, order = po.build_keys(hk, lam, P, intermediates=True)  # order does not depend on alpha
+    st, pil, off, rm, roff = po.build_keys_alpha(hk, lam, P, alpha)
     assert st == 0
     np_, B = po.sizes(n, lam, P)
     f = po.fulcrums(lam, P)
     lib = po.load()
+    a = po.alpha_ppm(alpha)
     part = np.array([lib.orc_partition_of(int(k[0]), np_) for k in hk])
     bkt = np.array([lib.orc_bucket_of(int(k[1]), f.ctypes.data) for k in hk])
     for p in range(np_):
         ps = int(off[p + 1] - off[p])
-        taken = np.zeros(ps, dtype=bool)
-        m = int((bs[p * B:(p + 1) * B] > 0).sum())
-        for r in range(m):
+        m = int(lib.orc_table_size(ps, a))
+        taken = np.zeros(m, dtype=bool)
+        nb = int((bs[p * B:(p + 1) * B] > 0).sum())
+        for r in range(nb):
             b = int(order[p * B + r])
             sel = (part == p) & (bkt == b)
             l1, l2 = hk[sel, 2], hk[sel, 3]
             V = int(pil[p + b * np_])
             found = None
-            for pilot in range(V // ps + 1):
-                pos = (_mix_np(l1, _mix_np(l2, pilot)) >> np.uint32(1)) % np.uint32(ps)
+            for pilot in range(V // m + 1):
+                pos = (_mix_np(l1, _mix_np(l2, pilot)) >> np.uint32(1)) % np.uint32(m)
                 if np.unique(pos).size != pos.size:
                     continue
-                d = np.arange(ps)
-                slots = (pos[None, :].astype(np.int64) + d[:, None]) % ps
+                d = np.arange(m)
+                slots = (pos[None, :].astype(np.int64) + d[:, None]) % m
                 ok = ~taken[slots].any(axis=1)
                 if ok.any():
-                    found = pilot * ps + int(np.argmax(ok))
+                    found = pilot * m + int(np.argmax(ok))
                     break
             assert found == V, (p, b, found, V)
-            pos = (_mix_np(l1, _mix_np(l2, V // ps)) >> np.uint32(1)) % np.uint32(ps)
-            taken[(pos.astype(np.int64) + V % ps) % ps] = True
-        assert taken.all()
-    _check_bijection(hk, pil, off, lam, P)
+            pos = (_mix_np(l1, _mix_np(l2, V // m)) >> np.uint32(1)) % np.uint32(m)
+            taken[(pos.astype(np.int64) + V % m) % m] = True
+        assert taken.sum() == ps
+        if alpha < 1.0:
+            used_high = np.nonzero(taken[ps:])[0]
+            holes = np.nonzero(~taken[:ps])[0]
+            assert used_high.size == holes.size
+            want = np.zeros(m - ps, np.uint32)
+            want[used_high] = holes
+            assert np.array_equal(rm[int(roff[p]):int(roff[p + 1])], want)
+    if alpha >= 1.0:
+        _check_bijection(hk, pil, off, lam, P)
+    else:
+        q = po.query_keys_alpha(hk, pil, off, rm, roff, lam, P, alpha)
+        assert np.unique(q).size == n and int(q.max()) < n
 
 
 def test_c1_golden_and_bijection():
@@ -503,3 +525,25 @@ def test_custom_table_build_is_a_bijection():
     bad = f.copy()
     bad[1000] = (B << 16) + 1
     assert po.build_keys_table(hk, P, B, bad)[0] == 1
+
+
+def test_device_outputs_keeps_its_buffers_alive():
+    """phobic_hip.device_outputs holds the buffer objects it was given, so a
+    buffer created inside the call cannot be freed while a build still writes
+    to it (a temporary freed mid-build once faulted a concurrent build)."""
+    import weakref
+
+    import phobic_hip as ph
+
+    class Buf:
+        def __init__(self, ptr):
+            self.ptr = ptr
+
+    bufs = [Buf(0x1000), Buf(0x2000), Buf(0x3000), Buf(0x4000)]
+    refs = [weakref.ref(b) for b in bufs]
+    out = ph.device_outputs(*bufs)
+    del bufs
+    assert all(r() is not None for r in refs)
+    assert (out.pilots, out.offsets, out.remap, out.remap_offsets) == (0x1000, 0x2000, 0x3000, 0x4000)
+    del out
+    assert all(r() is None for r in refs)
