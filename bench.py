@@ -525,6 +525,32 @@ def run_strings(args) -> dict:
     return res
 
 
+def run_host(args) -> dict:
+    """The host-buffer build (phobic_build_u64): keys in pageable host memory,
+    pilots, offsets and the remap table copied back to host memory, one
+    synchronous call per build, as MPHFbuilder::build."""
+    n, lam, P, alpha = args.n, args.lam, args.P, args.alpha
+    cfg = ph.make_config(lam, P, 0, alpha)
+    d = hip_rt.DeviceBuffer(n * 8)
+    ph.gen_splitmix64_device(args.key_seed, n, d)
+    hip_rt.synchronize()
+    keys = d.to_numpy(np.uint64)
+    del d
+    ctx = ph.Context(0)
+    r = ctx.build_u64(keys, cfg)  # warm-up: scratch, staging chunks
+    times = []
+    for _ in range(max(1, min(args.steps, 3))):
+        t0 = time.perf_counter()
+        r = ctx.build_u64(keys, cfg)
+        times.append(time.perf_counter() - t0)
+    best = min(times)
+    ctx.close()
+    return {"workload": f"phobic_build_u64 of {n} u64 keys from pageable host memory, outputs to host memory, "
+                        f"lambda {lam}, P {P}, alpha {alpha}",
+            "value": round(n / best / 1e6, 1), "unit": "Mkeys/s", "ms_per_build": round(best * 1e3, 2),
+            "stages_ms": {k: round(v, 3) for k, v in r.times.items()}}
+
+
 def bench_copy(args) -> int:
     """Copies of 1 GiB device to device, by the runtime (hipMemcpyAsync) and
     by the library's hand-written streaming kernel (phobic_copy_device): the
@@ -576,7 +602,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-verify", action="store_true")
-    ap.add_argument("--mode", choices=["all", "plain", "strings", "batch", "serve", "copy"], default="all")
+    ap.add_argument("--mode", choices=["all", "plain", "strings", "batch", "serve", "copy", "host"], default="all")
     ap.add_argument("--strings", type=int, default=200_000_000, help="string count of the string-keyed build")
     ap.add_argument("--sets", type=int, default=4096)
     ap.add_argument("--requests", type=int, default=48)
@@ -607,6 +633,10 @@ def main() -> int:
         res["value"] = res["p99_ms"]
         print(json.dumps(part_line("p99 latency", "ms", res, args, higher=False)), flush=True)
         return 0 if res["failed"] == 0 else 1
+    if args.mode == "host":
+        res = run_host(args)
+        print(json.dumps(part_line("MPHF build Mkeys/s, host buffers", "Mkeys/s", res, args)), flush=True)
+        return 0
     if args.mode == "strings":
         res = run_strings(args)
         print(json.dumps(part_line("MPHF build Mkeys/s, 200M strings", "Mkeys/s", res, args)), flush=True)
