@@ -204,15 +204,17 @@ int phobic_batch_destroy(phobic_batch* batch);
 /* ---- serving: continuous-batching build server ----
  * Requests queue in arrival order.  Whenever one of `slots` workers is idle,
  * the scheduler takes the oldest request that fits the HBM budget (a later
- * one may overtake a request that does not fit yet) and, when it is small
- * (at most 4e6 keys: u64 keys, or byte strings in host memory, with the
- * default OptBucketer table), every queued small request of the same
- * configuration (key kind, lambda, P, seed, alpha) with it, up to 64e6 keys
- * and 1024 requests: they are built together as one flattened launch whose
- * per-request outputs equal building each alone.  Larger requests run on a
- * slot of their own.  Each worker owns a context and stream, so admitted
- * launches run concurrently on the device.  Results stay in device memory
- * until phobic_server_release.  Host key buffers must stay valid until the
+ * one may overtake a request that does not fit yet) together with every
+ * queued request of the same configuration (key kind, lambda, P, seed,
+ * alpha) and size class: small requests (at most 4e6 keys) up to 64e6 keys
+ * in all, large ones up to 256e6, 1024 requests at most.  They are built as
+ * one flattened launch whose per-request outputs equal building each alone.
+ * Requests with a custom fulcrum table, and byte strings already in device
+ * memory, run alone.  Each worker owns a context and stream, so admitted
+ * launches run concurrently on the device; host inputs are uploaded through
+ * the worker context's pinned staging chunks (two of 64 MiB per worker,
+ * allocated by phobic_server_create).  Results stay in device memory until
+ * phobic_server_release.  Host key buffers must stay valid until the
  * request completes.
  * Memory: hbm_in_use counts the outputs kept for unreleased requests, the
  * staging, outputs and scratch growth charged to running launches, and the
@@ -344,6 +346,14 @@ int phobic_check_bijection_device(phobic_ctx* ctx, const uint32_t* d_pos, uint64
  * `stream`.  No counterpart in the reference: it is the known-size kernel
  * against which HBM traffic counters (FETCH_SIZE, WRITE_SIZE) are scaled. */
 int phobic_copy_device(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
+
+/* Host-to-device copy of `bytes` from host memory h_src (pageable) to d_dst
+ * on `stream`, through the context's two pinned staging chunks for copies of
+ * 16 MiB and more (the path the host-buffer builds and the server use);
+ * returns when the copy has completed.  Replaces the staging-buffer upload
+ * of the reference's host inputs (mphf_builder.h:231-238, a host-visible
+ * Vulkan buffer filled by memcpy). */
+int phobic_copy_to_device(phobic_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes, void* stream);
 
 /* ---- synthetic inputs (device-side generators used by benchmarks) ----
  * d_keys[i] = splitmix64 output number i+1 from state `seed`

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <vector>
 
 #include "../../include/phobic_hip.h"
@@ -47,6 +48,17 @@ void ctx_trim(phobic_ctx* ctx);
 // device's pool has handed out.
 uint64_t pool_bytes(uint64_t b);
 uint64_t pool_used_bytes();
+
+// Writes bytes [off, off + len) of a staged upload to dst (host memory).
+// Called from several threads at once, for disjoint ranges.
+using StageFill = std::function<void(char* dst, uint64_t off, uint64_t len)>;
+// The `bytes` bytes that fill produces, into dst (device memory), through the
+// context's two pinned host chunks: while one chunk's DMA runs on s, the
+// next chunk is filled.  Returns once fill has produced every byte (the
+// caller's host data is then no longer read); the copies are ordered on s.
+int stage_to_device(phobic_ctx* ctx, void* dst, uint64_t bytes, const StageFill& fill, hipStream_t s);
+// Allocates the context's pinned chunks now instead of at first use.
+int ctx_prepare_staging(phobic_ctx* ctx);
 
 // Looks up (once per device; the device must be current) the device address
 // of the fixed scratch array of the allocation-free entry point.

@@ -4,13 +4,15 @@
 //
 // Requests queue in arrival order.  Whenever a worker slot is idle the
 // scheduler forms a job from the oldest request that fits the HBM budget
-// (backfill: a later request may overtake one that does not fit yet).  A
-// small request takes every queued small request of its configuration with
-// it: the job is one flattened launch over all of their keys (a group, built
-// by phobic::group_build), whose per-request outputs equal building each
-// alone.  Large requests run alone.  Every slot owns a phobic_ctx (its own
-// stream and scratch), so jobs run concurrently on the device; results stay
-// in device memory until released.
+// (backfill: a later request may overtake one that does not fit yet).  The
+// request takes every queued request of its configuration and size class
+// with it: the job is one flattened launch over all of their keys (a group,
+// built by phobic::group_build), whose per-request outputs equal building
+// each alone.  A request with a custom bucketer table, or device-resident
+// strings, runs alone.  Every slot owns a phobic_ctx (its own stream and
+// scratch), so jobs run concurrently on the device; results stay in device
+// memory until released.  Host inputs are uploaded through the slot
+// context's pinned staging chunks (allocated when the server starts).
 //
 // Memory accounting (all in the pool's 2 MiB units):
 //   kept      outputs of completed, unreleased requests (one block per job,
@@ -42,9 +44,14 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr uint64_t kGroupMaxKeys = 4000000;  // requests up to this size share launches
-constexpr uint64_t kGroupKeys = 64000000;    // keys of one shared launch at most
-constexpr uint32_t kGroupSets = 1024;        // requests of one shared launch at most
+// Queued requests of one configuration share launches.  Small and large
+// requests are grouped apart, so that a small request never waits for a
+// large one's build: small groups hold up to kGroupKeys keys, large ones
+// (requests above kSmallKeys each) up to kGroupKeysLarge.
+constexpr uint64_t kSmallKeys = 4000000;
+constexpr uint64_t kGroupKeys = 64000000;
+constexpr uint64_t kGroupKeysLarge = 256000000;
+constexpr uint32_t kGroupSets = 1024;  // requests of one shared launch at most
 
 uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
@@ -80,6 +87,19 @@ uint64_t single_in_bytes(const Request& r) {
   return r.req.n * 8;
 }
 
+// Host inputs below this size are copied directly (pageable memory); larger
+// ones go through the slot context's pinned staging chunks.
+constexpr uint64_t kDirectCopyMax = 1ull << 20;
+
+int upload(phobic_ctx* ctx, void* dst, const void* src, uint64_t bytes, hipStream_t s) {
+  if (!bytes) return PHOBIC_OK;
+  if (bytes < kDirectCopyMax)
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) == hipSuccess ? PHOBIC_OK : PHOBIC_E_HIP;
+  return phobic::stage_to_device(
+      ctx, dst, bytes,
+      [src](char* d, uint64_t lo, uint64_t len) { std::memcpy(d, static_cast<const char*>(src) + lo, len); }, s);
+}
+
 // Output block of a group: pilots | offsets | remap | remap offsets | status
 // over the flattened build.
 uint64_t group_out_bytes(const phobic::GroupLayout& gl) {
@@ -105,15 +125,18 @@ struct Slot {
 };
 
 bool groupable(const Request& r) {
-  return r.req.n > 0 && r.req.n <= kGroupMaxKeys && !r.cfg.fulcrums &&
+  return r.req.n > 0 && !r.cfg.fulcrums &&
          (r.req.kind == PHOBIC_KEYS_U64 || (r.req.kind == PHOBIC_KEYS_BYTES && !r.req.keys_on_device));
 }
+
+bool large(const Request& r) { return r.req.n > kSmallKeys; }
 
 double alpha_of(const phobic_config& c) { return c.alpha == 0.0 ? 1.0 : c.alpha; }
 
 bool same_class(const Request& a, const Request& b) {
   return a.req.kind == b.req.kind && a.cfg.lambda == b.cfg.lambda && a.cfg.partition_size == b.cfg.partition_size &&
-         a.cfg.buckets == b.cfg.buckets && a.cfg.seed == b.cfg.seed && alpha_of(a.cfg) == alpha_of(b.cfg);
+         a.cfg.buckets == b.cfg.buckets && a.cfg.seed == b.cfg.seed && alpha_of(a.cfg) == alpha_of(b.cfg) &&
+         large(a) == large(b);
 }
 
 }  // namespace
@@ -168,8 +191,9 @@ std::shared_ptr<Job> form_job(phobic_server* s, std::deque<std::shared_ptr<Reque
     j->reqs.push_back(*it);
     if (groupable(**it)) {
       uint64_t keys = (*it)->req.n;
+      const uint64_t cap = large(**it) ? kGroupKeysLarge : kGroupKeys;
       for (auto jt = std::next(it); jt != s->queue.end() && j->reqs.size() < kGroupSets; ++jt)
-        if (groupable(**jt) && same_class(**it, **jt) && keys + (*jt)->req.n <= kGroupKeys) {
+        if (groupable(**jt) && same_class(**it, **jt) && keys + (*jt)->req.n <= cap) {
           j->reqs.push_back(*jt);
           keys += (*jt)->req.n;
         }
@@ -290,9 +314,9 @@ int run_single(phobic_ctx* ctx, Request& r, hipStream_t stream) {
     const uint64_t ob = r.req.kind == PHOBIC_KEYS_BYTES ? (n + 1) * 8 : 0;
     if (phobic::dev_alloc(&staging, single_in_bytes(r), stream) != hipSuccess) return done(PHOBIC_E_OOM);
     uint8_t* q = static_cast<uint8_t*>(staging);
-    if (kb && hipMemcpyAsync(q, r.req.keys, kb, hipMemcpyHostToDevice, stream) != hipSuccess) return done(PHOBIC_E_HIP);
-    if (ob && hipMemcpyAsync(q + align256(kb), r.req.offsets, ob, hipMemcpyHostToDevice, stream) != hipSuccess)
-      return done(PHOBIC_E_HIP);
+    int ust = upload(ctx, q, r.req.keys, kb, stream);
+    if (!ust) ust = upload(ctx, q + align256(kb), r.req.offsets, ob, stream);
+    if (ust) return done(ust);
     keys = q;
     offs = reinterpret_cast<const uint64_t*>(q + align256(kb));
   }
@@ -334,27 +358,61 @@ int run_group(phobic_ctx* ctx, Job& j, hipStream_t stream) {
   uint64_t key_bytes = 0;
   for (const auto& r : j.reqs) key_bytes += bytes ? r->req.bytes_len : r->req.n * 8;
   uint64_t* d_offs = reinterpret_cast<uint64_t*>(q + align256(key_bytes));
-  std::vector<uint64_t> offs;
-  try {
-    if (bytes) offs.resize(gl.n + 1);
-  } catch (...) {
-    return fail(PHOBIC_E_OOM);
+  // Each request's keys at `at[s]` of the staging area.  Runs of requests in
+  // host memory are gathered into the pinned chunks and uploaded together;
+  // device-resident u64 keys are copied on the device.
+  std::vector<uint64_t> at(gl.sets + 1, 0);
+  std::vector<const char*> src(gl.sets);
+  for (uint32_t k = 0; k < gl.sets; k++) {
+    const Request& r = *j.reqs[k];
+    const uint64_t kb = bytes ? r.req.offsets[r.req.n] - r.req.offsets[0] : r.req.n * 8;
+    src[k] = static_cast<const char*>(r.req.keys) + (bytes ? r.req.offsets[0] : 0);
+    at[k + 1] = at[k] + kb;
   }
-  uint64_t at = 0, key = 0;
-  for (const auto& r : j.reqs) {
-    const uint64_t kb = bytes ? r->req.offsets[r->req.n] - r->req.offsets[0] : r->req.n * 8;
-    const void* src = bytes ? static_cast<const uint8_t*>(r->req.keys) + r->req.offsets[0] : r->req.keys;
-    const hipMemcpyKind dir = r->req.keys_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    if (kb && hipMemcpyAsync(q + at, src, kb, dir, stream) != hipSuccess) return fail(PHOBIC_E_HIP);
-    if (bytes)
-      for (uint64_t i = 0; i < r->req.n; i++) offs[key + i] = at + (r->req.offsets[i] - r->req.offsets[0]);
-    at += kb;
-    key += r->req.n;
+  for (uint32_t k = 0; k < gl.sets;) {
+    if (j.reqs[k]->req.keys_on_device) {
+      const uint64_t kb = at[k + 1] - at[k];
+      if (kb && hipMemcpyAsync(q + at[k], src[k], kb, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+        return fail(PHOBIC_E_HIP);
+      k++;
+      continue;
+    }
+    uint32_t e = k + 1;
+    while (e < gl.sets && !j.reqs[e]->req.keys_on_device) e++;
+    // requests [k, e): bytes [at[k], at[e]) of the staging area
+    const uint64_t base = at[k];
+    auto gather = [&, k, e, base](char* d, uint64_t off, uint64_t len) {
+      uint32_t t = (uint32_t)(std::upper_bound(at.begin() + k, at.begin() + e + 1, base + off) - at.begin()) - 1;
+      for (uint64_t done = 0; done < len; t++) {
+        const uint64_t from = base + off + done - at[t];
+        const uint64_t take = std::min(at[t + 1] - at[t] - from, len - done);
+        std::memcpy(d + done, src[t] + from, take);
+        done += take;
+      }
+    };
+    const int ust = at[e] > at[k] ? phobic::stage_to_device(ctx, q + base, at[e] - base, gather, stream) : PHOBIC_OK;
+    if (ust) return fail(ust);
+    k = e;
   }
   if (bytes) {
-    offs[gl.n] = at;
-    if (hipMemcpyAsync(d_offs, offs.data(), (gl.n + 1) * 8, hipMemcpyHostToDevice, stream) != hipSuccess)
-      return fail(PHOBIC_E_HIP);
+    // the group's n + 1 offsets: request s's offsets rebased onto at[s],
+    // then the total, produced straight into the pinned chunks
+    auto rebase = [&](char* d, uint64_t off, uint64_t len) {
+      uint64_t* o = reinterpret_cast<uint64_t*>(d);
+      const uint64_t i0 = off / 8, cnt = len / 8;
+      uint32_t t = (uint32_t)(std::upper_bound(gl.key_off.begin(), gl.key_off.end(), i0) - gl.key_off.begin()) - 1;
+      for (uint64_t i = i0; i < i0 + cnt; i++) {
+        while (t < gl.sets && i >= gl.key_off[t + 1]) t++;
+        if (t == gl.sets) {
+          o[i - i0] = at[gl.sets];
+          continue;
+        }
+        const uint64_t* ro = j.reqs[t]->req.offsets;
+        o[i - i0] = at[t] + (ro[i - gl.key_off[t]] - ro[0]);
+      }
+    };
+    const int ust = phobic::stage_to_device(ctx, d_offs, (gl.n + 1) * 8, rebase, stream);
+    if (ust) return fail(ust);
   }
   uint8_t* p = static_cast<uint8_t*>(block->ptr);
   const uint64_t entries = (uint64_t)gl.np + gl.sets;
@@ -488,7 +546,10 @@ int phobic_server_create(const phobic_server_config* cfg, phobic_server** out) {
     return PHOBIC_E_OOM;
   }
   for (auto& sl : s->slots) {
-    const int st = phobic_ctx_create(cfg->device, &sl.ctx);
+    // each slot's pinned staging chunks are allocated now, not by the first
+    // request that uploads host keys
+    int st = phobic_ctx_create(cfg->device, &sl.ctx);
+    if (st == PHOBIC_OK) st = phobic::ctx_prepare_staging(sl.ctx);
     if (st != PHOBIC_OK) {
       for (auto& x : s->slots)
         if (x.ctx) phobic_ctx_destroy(x.ctx);

@@ -138,6 +138,7 @@ EXPORTS = [
     "phobic_eval_keys_device",
     "phobic_check_bijection_device",
     "phobic_copy_device",
+    "phobic_copy_to_device",
     "phobic_gen_splitmix64_device",
     "phobic_gen_strings_device",
 ]
@@ -196,6 +197,7 @@ def load() -> ctypes.CDLL:
         "phobic_eval_keys_device": (I, [P, P, U64, cfgp, U32, op, P, P]),
         "phobic_check_bijection_device": (I, [P, P, U64, U64, P, P, P]),
         "phobic_copy_device": (I, [P, P, U64, P]),
+        "phobic_copy_to_device": (I, [P, P, P, U64, P]),
         "phobic_gen_splitmix64_device": (I, [U64, U64, P, P]),
         "phobic_gen_strings_device": (I, [U64, U64, P, P, P]),
     }
@@ -400,6 +402,14 @@ class Context:
         if st:
             raise PhobicError(st, "last_stage_times")
         return t.as_dict()
+
+    def copy_to_device(self, d_dst, host: np.ndarray, stream=None) -> None:
+        """phobic_copy_to_device: a host array into device memory through the
+        context's pinned staging chunks; returns when the copy is done."""
+        host = np.ascontiguousarray(host)
+        st = self.lib.phobic_copy_to_device(self.h, _ptr(d_dst), host.ctypes.data, host.nbytes, _stream(stream))
+        if st:
+            raise PhobicError(st, "copy_to_device")
 
     def eval_u64_device(self, d_keys, count, cfg, np_, mphf: Outputs, d_out, stream=None) -> None:
         st = self.lib.phobic_eval_u64_device(self.h, _ptr(d_keys), count, ctypes.byref(cfg), np_,

@@ -8,6 +8,7 @@ import time
 import numpy as np
 import pytest
 
+import hip_rt
 import phobic_hip as ph
 import pyoracle as po
 
@@ -108,6 +109,61 @@ def test_queued_small_requests_share_launches():
     s = srv.stats()
     assert s["grouped_launches"] >= 3 and s["grouped_requests"] >= 30
     for rid in ids + [big]:
+        srv.release(rid)
+    srv.close()
+
+
+def test_large_requests_group_among_themselves():
+    """Large requests (above 4e6 keys) of one configuration that queue behind
+    a busy slot are built as one launch of their own, apart from the small
+    requests of that configuration; host keys and device-resident keys mix
+    in one group, and every result is the oracle's."""
+    cfg = ph.make_config(8.0, 2048, 0, 0.99)
+    sizes = [4_500_001, 6_100_000, 5_000_000, 120_000, 90_000, 4_000_000]
+    keys = [po.splitmix64(400 + i, n) for i, n in enumerate(sizes)]
+    d_keys = hip_rt.DeviceBuffer.from_numpy(keys[1])  # device-resident, between two host requests
+    big = po.splitmix64(78, 60_000_000)
+    srv = ph.Server(slots=1, hbm_budget=64 << 30)
+    blocker = srv.submit_u64(big, ph.make_config(7.0, 2048))  # the others queue behind it
+    ids = [srv.submit_u64(d_keys, cfg, sizes[1]) if i == 1 else srv.submit_u64(k, cfg) for i, k in enumerate(keys)]
+    assert srv.wait(blocker)[0] == 0
+    for rid, k in zip(ids, keys):
+        assert srv.wait(rid)[0] == 0
+        check_against_oracle(srv.fetch(rid), ("u64", k, cfg, 8.0, 0.99))
+    s = srv.stats()
+    # one launch of the three large requests, one of the three small ones
+    assert s["grouped_launches"] == 2 and s["grouped_requests"] == 6
+    for rid in ids + [blocker]:
+        srv.release(rid)
+    srv.close()
+
+
+def test_grouped_host_strings_rebased_through_staging():
+    """A group of host string requests larger than the direct-copy limit
+    (their bytes and offsets go through the pinned staging chunks, the
+    offsets rebased per request while staged), with requests whose offsets
+    do not start at 0, gives the oracle's results."""
+    cfg = ph.make_config(6.0, 2048, 0, 0.99)
+    rng = np.random.default_rng(11)
+    reqs = []
+    for n in [600_000, 1_500_000, 35_000, 900_000]:
+        skip = int(rng.integers(0, 5000))  # bytes before the first key
+        offs = np.zeros(n + 1, np.uint64)
+        offs[1:] = np.cumsum(rng.integers(8, 65, n))
+        offs += skip
+        data = rng.integers(0x20, 0x7F, int(offs[-1]) + int(rng.integers(0, 100)), dtype=np.uint8)
+        reqs.append(("bytes", (data, offs), cfg, 6.0, 0.99))
+    big = po.splitmix64(79, 60_000_000)
+    srv = ph.Server(slots=1, hbm_budget=64 << 30)
+    blocker = srv.submit_u64(big, ph.make_config(7.0, 2048))  # the others queue behind it
+    ids = [srv.submit_bytes(r[1][0], r[1][1], cfg) for r in reqs]
+    assert srv.wait(blocker)[0] == 0
+    for rid, req in zip(ids, reqs):
+        assert srv.wait(rid)[0] == 0
+        check_against_oracle(srv.fetch(rid), req)
+    s = srv.stats()
+    assert s["grouped_launches"] == 1 and s["grouped_requests"] == 4
+    for rid in ids + [blocker]:
         srv.release(rid)
     srv.close()
 
