@@ -77,8 +77,9 @@ constexpr uint32_t kLoadBatch = 8;
 // half full); below it the full scan alone is cheaper
 constexpr uint32_t kWin0MinKeys = 6;
 
-// smallest table for which mod_f32 is exact (quotients below 2^22)
-constexpr uint32_t kF32ModMin = 512;
+// smallest table for which mod_magic's 24-bit product is exact (quotients
+// below 2^22)
+constexpr uint32_t kMagicModMin = 512;
 
 template <int CTRL, int ROWS, bool BOUND>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
@@ -118,21 +119,20 @@ __device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t M, uint32_t d) 
   return (uint32_t)__umul64hi(M * a, (uint64_t)d);
 }
 
-// a mod d for a < 2^31 and 512 <= d < 2^24 without 32-bit integer
-// multiplies (quarter rate): the f32 quotient estimate q has a relative
-// error below 3 * 2^-24, so for a / d < 2^22 it is off by at most one, and
-// r = a - q*d (mod 2^32, a 24x24-bit product) is the remainder plus d, the
-// remainder, or the remainder minus d (wrapped).  Of r, r + d and r - d the
-// smallest as unsigned is then the remainder: the others are at least d or
-// wrapped past 2^32 - 2d.
-__device__ __forceinline__ uint32_t mod_f32(uint32_t a, float rd, uint32_t d) {
-  const uint32_t q = (uint32_t)(__uint2float_rn(a) * rd);
-  // the 24x24-bit product as one full-rate instruction (the compiler
-  // otherwise emits a mask and a quarter-rate v_mul_lo_u32 in some loops)
+// a mod d for a < 2^31 and 512 <= d < 2^24 by a multiply with a magic
+// number (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", 1994, Theorem 4.2): with l = ceil(log2 d) and
+// m = ceil(2^(31+l) / d), m*d - 2^(31+l) < d <= 2^l, so floor(a / d) =
+// floor(m*a / 2^(31+l)) = mulhi(a, m) >> (l - 1) for every a < 2^31, and
+// m < 2^32 (m = 2^31 when d is a power of two).  The quotient is below 2^22,
+// so q*d is one 24x24-bit product.
+__device__ __forceinline__ uint32_t mod_magic(uint32_t a, uint32_t m, uint32_t shift, uint32_t d) {
+  const uint32_t q = __umulhi(a, m) >> shift;
+  // the 24x24-bit product as one instruction (the compiler otherwise emits a
+  // mask and a v_mul_lo_u32 in some loops)
   uint32_t qd;
   asm("v_mul_u32_u24 %0, %1, %2" : "=v"(qd) : "s"(d), "v"(q));
-  const uint32_t r = a - qd;
-  return min(min(r, r + d), r - d);
+  return a - qd;
 }
 
 __device__ __forceinline__ uint64_t low_mask(uint32_t bits) {  // bits >= 64 -> all ones
@@ -481,9 +481,20 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
 
   // 5. pilot search over a table of tm slots (tm = ps unless alpha < 1)
   const uint32_t tm = table_size(ps, a.a_ppm);
-  const uint64_t M = ~0ull / (tm ? tm : 1) + 1;
-  const bool f32mod = tm >= kF32ModMin;
-  const float rd = (float)(1.0 / (double)(tm ? tm : 1));
+  const bool magic = tm >= kMagicModMin;
+  // One 64-bit constant serves either remainder: fastmod's M for small
+  // tables; for the others, mod_magic's m = ceil(2^(31+l) / tm) (low word)
+  // and shift l - 1 (high word), l = ceil(log2 tm).  The f64 quotient is
+  // within 2^-21 of the exact one, and a quotient that is not an integer
+  // lies at least 1 / tm >= 2^-13 below the next integer, so its ceiling is
+  // exact.
+  uint64_t modc;
+  if (magic) {
+    const uint32_t l = 32 - __builtin_clz(tm - 1);
+    modc = (uint64_t)(uint32_t)ceil(ldexp(1.0, 31 + (int)l) / (double)tm) | (uint64_t)(l - 1) << 32;
+  } else {
+    modc = ~0ull / (tm ? tm : 1) + 1;
+  }
   const uint32_t L = 2 * tm + 128;  // valid bits of E
   // largest pilot whose offsets [0, tm) all keep V below 2^31
   const uint32_t pilot_full = tm ? (uint32_t)(0x80000000ull / tm) - 1 : 0;
@@ -492,7 +503,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // packed slot of key kv for pilot pl: (hash(l1, hash(l2, pl)) >> 1) mod tm
   auto slot = [&](uint64_t kv, uint32_t pl) {
     const uint32_t h = mix32((uint32_t)kv, mix32((uint32_t)(kv >> 32), pl)) >> 1;
-    return pack_slot(f32mod ? mod_f32(h, rd, tm) : fastmod(h, M, tm));
+    return pack_slot(magic ? mod_magic(h, (uint32_t)modc, (uint32_t)(modc >> 32), tm) : fastmod(h, modc, tm));
   };
 
   struct Bucket {
