@@ -172,6 +172,41 @@ __device__ __forceinline__ void or_windows(const char* eb, uint32_t pk, uint32_t
     for (int h = 0; h < H; h++) acc[h] |= __builtin_amdgcn_alignbit(hi[i][h], lo[i][h], x[i]);
 }
 
+// Each even lane's packed slot with its odd neighbour's in the high half
+// (DPP quad_perm [1,0,3,2]); packed slots of tables up to 4096 slots are
+// below 2^14.
+__device__ __forceinline__ uint32_t pair_pack(uint32_t pk) {
+  return pk | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pk, 0xB1, 0xF, 0xF, false) << 16;
+}
+
+// ORs into acc[h] the windows of the 2 NP slots held by lanes lane0 ..
+// lane0 + 2 NP - 1 (lane0 even) of a pair_pack register: one v_readlane
+// fetches two slots.  The first slot's bit shift is the low 5 bits of the
+// pair itself, the second's those of the pair >> 16.
+template <int H, uint32_t NP>
+__device__ __forceinline__ void or_pairs(const char* eb, uint32_t pk2, uint32_t lane0, uint32_t (&acc)[H]) {
+  uint32_t x[NP], lo[NP][2][H], hi[NP][2][H];
+#pragma unroll
+  for (uint32_t i = 0; i < NP; i++) {
+    x[i] = __builtin_amdgcn_readlane(pk2, lane0 + 2 * i);
+    const uint32_t* ew0 = reinterpret_cast<const uint32_t*>(eb + ((x[i] & 0xFFFFu) >> 5));
+    const uint32_t* ew1 = reinterpret_cast<const uint32_t*>(eb + (x[i] >> 21));
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+      lo[i][0][h] = ew0[64 * h];
+      hi[i][0][h] = ew0[64 * h + 1];
+      lo[i][1][h] = ew1[64 * h];
+      hi[i][1][h] = ew1[64 * h + 1];
+    }
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < NP; i++)
+#pragma unroll
+    for (int h = 0; h < H; h++)
+      acc[h] |= __builtin_amdgcn_alignbit(hi[i][0][h], lo[i][0][h], x[i]) |
+                __builtin_amdgcn_alignbit(hi[i][1][h], lo[i][1][h], x[i] >> 16);
+}
+
 // The windows of all s slots of a bucket: lanes [gbase, gbase + s) of pk[0]
 // when C = 1 (4 or 2, then 2, then 1 at a time, so no slot is read twice), or
 // lane k & 63 of pk[k >> 6] when C > 1 (2 at a time, the last slot repeated
@@ -222,13 +257,24 @@ __device__ __forceinline__ void or_bucket(const char* eb, const uint32_t (&pk)[C
 // most 2048 H slots): the smallest offset at which the s slots of lane
 // group gbase (C = 1) are all free, or ~0u.
 template <int H>
-__device__ __forceinline__ uint32_t scan_pass(const uint32_t* Elane, uint32_t pk, uint32_t gbase, uint32_t s,
-                                              const uint32_t (&vfull)[H]) {
+__device__ __forceinline__ uint32_t scan_pass(const uint32_t* Elane, uint32_t pk, uint32_t pk2, uint32_t gbase,
+                                              uint32_t s, const uint32_t (&vfull)[H]) {
   uint32_t acc[H];
 #pragma unroll
   for (int h = 0; h < H; h++) acc[h] = 0;
-  const uint32_t pka[1] = {pk};
-  or_bucket<1, H>(reinterpret_cast<const char*>(Elane), pka, gbase, s, acc);
+  const char* eb = reinterpret_cast<const char*>(Elane);
+  if constexpr (H < 4) {  // two slots per lane read (gbase is a multiple of 4)
+    uint32_t k = 0;
+    for (; k + 4 <= s; k += 4) or_pairs<H, 2>(eb, pk2, gbase + k, acc);
+    if (s & 2) {
+      or_pairs<H, 1>(eb, pk2, gbase + k, acc);
+      k += 2;
+    }
+    if (s & 1) or_windows<H, 1>(eb, pk, gbase + k, acc);
+  } else {
+    const uint32_t pka[1] = {pk};
+    or_bucket<1, H>(eb, pka, gbase, s, acc);
+  }
   uint32_t fr[H];
   uint64_t any[H], all = 0;
 #pragma unroll
@@ -572,6 +618,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t G = 64u >> lg;
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
       uint32_t pk[C];  // packed slots
+      uint32_t pk2 = 0;  // pk[0] paired (pair_pack), in the lean loop
       uint32_t pilot = 0, dsel = 0;
       bool dupchk = false, done = false;  // done: placed and marked by the lean loop
       if constexpr (C == 1) {
@@ -609,11 +656,14 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         if (win0) {
           for (; pilot < fast_end; pilot++) {
             const uint32_t gi = pilot & (G - 1);
-            if (gi == 0) pk[0] = slot(key[0], pilot + g);
+            if (gi == 0) {
+              pk[0] = slot(key[0], pilot + g);
+              pk2 = pair_pack(pk[0]);
+            }
             const uint64_t t0 = g == gi ? window64(E, pk[0]) : 0;
             const uint64_t free0 = ~(((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0)) &
                                    low_mask(tm);
-            uint32_t d = free0 ? (uint32_t)__builtin_ctzll(free0) : scan_pass<H>(Elane, pk[0], gi << lg, s, vfull);
+            uint32_t d = free0 ? (uint32_t)__builtin_ctzll(free0) : scan_pass<H>(Elane, pk[0], pk2, gi << lg, s, vfull);
             if (d == ~0u) continue;
             if (mark_distinct(gi, d)) {
               dsel = d;
@@ -624,8 +674,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         } else {
           for (; pilot < fast_end; pilot++) {
             const uint32_t gi = pilot & (G - 1);
-            if (gi == 0) pk[0] = slot(key[0], pilot + g);
-            const uint32_t d = scan_pass<H>(Elane, pk[0], gi << lg, s, vfull);
+            if (gi == 0) {
+              pk[0] = slot(key[0], pilot + g);
+              pk2 = pair_pack(pk[0]);
+            }
+            const uint32_t d = scan_pass<H>(Elane, pk[0], pk2, gi << lg, s, vfull);
             if (d == ~0u) continue;
             if (mark_distinct(gi, d)) {
               dsel = d;
