@@ -264,13 +264,15 @@ __device__ __forceinline__ uint32_t scan_pass(const uint32_t* Elane, uint32_t pk
   for (int h = 0; h < H; h++) acc[h] = 0;
   const char* eb = reinterpret_cast<const char*>(Elane);
   if constexpr (H < 4) {  // two slots per lane read (gbase is a multiple of 4)
-    uint32_t k = 0;
-    for (; k + 4 <= s; k += 4) or_pairs<H, 2>(eb, pk2, gbase + k, acc);
+    // the lane index itself is the loop variable
+    const uint32_t lend = gbase + (s & ~3u);
+    uint32_t l = gbase;
+    for (; l < lend; l += 4) or_pairs<H, 2>(eb, pk2, l, acc);
     if (s & 2) {
-      or_pairs<H, 1>(eb, pk2, gbase + k, acc);
-      k += 2;
+      or_pairs<H, 1>(eb, pk2, l, acc);
+      l += 2;
     }
-    if (s & 1) or_windows<H, 1>(eb, pk, gbase + k, acc);
+    if (s & 1) or_windows<H, 1>(eb, pk, l, acc);
   } else {
     const uint32_t pka[1] = {pk};
     or_bucket<1, H>(eb, pka, gbase, s, acc);
