@@ -646,17 +646,20 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
             for (uint32_t x = q1 + tm; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
           }
           lds_sync();
-          if (!__any(c)) return true;
+          if (__ballot(c) == 0) return 1u;
           if (mine) {
             atomicAnd(&E[q >> 5], ~m0);
             atomicAnd(&E[q1 >> 5], ~m1);
             for (uint32_t x = q1 + tm; x < L; x += tm) atomicAnd(&E[x >> 5], ~(1u << (x & 31)));
           }
           lds_sync();
-          return false;
+          return 0u;
         };
+        // The pilot loops have one exit, at the bottom, so that the
+        // compiler keeps no exit flags in them.
         if (win0) {
-          for (; pilot < fast_end; pilot++) {
+          bool more = pilot < fast_end;
+          while (more) {
             const uint32_t gi = pilot & (G - 1);
             if (gi == 0) {
               pk[0] = slot(key[0], pilot + g);
@@ -665,28 +668,30 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
             const uint64_t t0 = g == gi ? window64(E, pk[0]) : 0;
             const uint64_t free0 = ~(((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0)) &
                                    low_mask(tm);
-            uint32_t d = free0 ? (uint32_t)__builtin_ctzll(free0) : scan_pass<H>(Elane, pk[0], pk2, gi << lg, s, vfull);
-            if (d == ~0u) continue;
-            if (mark_distinct(gi, d)) {
-              dsel = d;
-              done = true;
-              break;
-            }
+            const uint32_t d =
+                free0 ? (uint32_t)__builtin_ctzll(free0) : scan_pass<H>(Elane, pk[0], pk2, gi << lg, s, vfull);
+            uint32_t ok = 0;  // 1: placed and marked (wave-uniform)
+            if (d != ~0u) ok = mark_distinct(gi, d);
+            dsel = d;
+            pilot += 1 - ok;
+            done = ok != 0;
+            more = ok == 0 && pilot < fast_end;
           }
         } else {
-          for (; pilot < fast_end; pilot++) {
+          bool more = pilot < fast_end;
+          while (more) {
             const uint32_t gi = pilot & (G - 1);
             if (gi == 0) {
               pk[0] = slot(key[0], pilot + g);
               pk2 = pair_pack(pk[0]);
             }
             const uint32_t d = scan_pass<H>(Elane, pk[0], pk2, gi << lg, s, vfull);
-            if (d == ~0u) continue;
-            if (mark_distinct(gi, d)) {
-              dsel = d;
-              done = true;
-              break;
-            }
+            uint32_t ok = 0;  // 1: placed and marked (wave-uniform)
+            if (d != ~0u) ok = mark_distinct(gi, d);
+            dsel = d;
+            pilot += 1 - ok;
+            done = ok != 0;
+            more = ok == 0 && pilot < fast_end;
           }
         }
       }
