@@ -183,7 +183,7 @@ __device__ __forceinline__ uint32_t pair_pack(uint32_t pk) {
 // lane0 + 2 NP - 1 (lane0 even) of a pair_pack register: one v_readlane
 // fetches two slots.  The first slot's bit shift is the low 5 bits of the
 // pair itself, the second's those of the pair >> 16.
-template <int H, uint32_t NP>
+template <int H, uint32_t NP, bool kSet = false>
 __device__ __forceinline__ void or_pairs(const char* eb, uint32_t pk2, uint32_t lane0, uint32_t (&acc)[H]) {
   uint32_t x[NP], lo[NP][2][H], hi[NP][2][H];
 #pragma unroll
@@ -202,9 +202,11 @@ __device__ __forceinline__ void or_pairs(const char* eb, uint32_t pk2, uint32_t 
 #pragma unroll
   for (uint32_t i = 0; i < NP; i++)
 #pragma unroll
-    for (int h = 0; h < H; h++)
-      acc[h] |= __builtin_amdgcn_alignbit(hi[i][0][h], lo[i][0][h], x[i]) |
-                __builtin_amdgcn_alignbit(hi[i][1][h], lo[i][1][h], x[i] >> 16);
+    for (int h = 0; h < H; h++) {
+      const uint32_t w = __builtin_amdgcn_alignbit(hi[i][0][h], lo[i][0][h], x[i]) |
+                         __builtin_amdgcn_alignbit(hi[i][1][h], lo[i][1][h], x[i] >> 16);
+      acc[h] = (kSet && i == 0) ? w : acc[h] | w;  // kSet: the first pair sets acc
+    }
 }
 
 // The windows of all s slots of a bucket: lanes [gbase, gbase + s) of pk[0]
@@ -260,20 +262,23 @@ template <int H>
 __device__ __forceinline__ uint32_t scan_pass(const uint32_t* Elane, uint32_t pk, uint32_t pk2, uint32_t gbase,
                                               uint32_t s, const uint32_t (&vfull)[H]) {
   uint32_t acc[H];
-#pragma unroll
-  for (int h = 0; h < H; h++) acc[h] = 0;
   const char* eb = reinterpret_cast<const char*>(Elane);
   if constexpr (H < 4) {  // two slots per lane read (gbase is a multiple of 4)
-    // the lane index itself is the loop variable
-    const uint32_t lend = gbase + (s & ~3u);
-    uint32_t l = gbase;
+    // s >= 2: the first pair sets the accumulators; then the lane index
+    // itself is the loop variable
+    or_pairs<H, 1, true>(eb, pk2, gbase, acc);
+    const uint32_t rest = s - 2;
+    uint32_t l = gbase + 2;
+    const uint32_t lend = l + (rest & ~3u);
     for (; l < lend; l += 4) or_pairs<H, 2>(eb, pk2, l, acc);
-    if (s & 2) {
+    if (rest & 2) {
       or_pairs<H, 1>(eb, pk2, l, acc);
       l += 2;
     }
-    if (s & 1) or_windows<H, 1>(eb, pk, l, acc);
+    if (rest & 1) or_windows<H, 1>(eb, pk, l, acc);
   } else {
+#pragma unroll
+    for (int h = 0; h < H; h++) acc[h] = 0;
     const uint32_t pka[1] = {pk};
     or_bucket<1, H>(eb, pka, gbase, s, acc);
   }
