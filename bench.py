@@ -380,7 +380,10 @@ def run_serve(args) -> dict:
     uploaded first, as the server uploads them).  Served throughput
     cannot exceed the trace's offered load (keys over the arrival span);
     the same requests submitted at once (`burst`) measure the server's
-    capacity against the sum of the sequential build times."""
+    capacity against the sum of the sequential build times.  The sequential
+    path runs the trace twice and keeps each request's faster time; the
+    server runs it three times each way after a warm-up pass, and the pass
+    of median span is reported."""
     R = args.requests
     arrivals, reqs = serve_trace(R)
     total = sum(r[1] for r in reqs)
@@ -393,30 +396,38 @@ def run_serve(args) -> dict:
     warm = hip_rt.DeviceBuffer(32 << 20)  # the context's pinned chunks exist before timing, as a server's do
     ctx.copy_to_device(warm, np.zeros(32 << 20, np.uint8), stream)
     del warm
-    build_s, upload_s = [], []
-    for kind, n, payload, cfg in reqs:
-        np_, B = ph.sizes(cfg, n)
-        rc = ph.remap_capacity(cfg, n)
-        dp, do, ds = hip_rt.DeviceBuffer(max(np_ * B, 1) * 4), hip_rt.DeviceBuffer((np_ + 1) * 4), hip_rt.DeviceBuffer(4)
-        drm = hip_rt.DeviceBuffer(max(rc, 1) * 4) if rc else None
-        dro = hip_rt.DeviceBuffer((np_ + 1) * 4) if rc else None
-        outs = ph.device_outputs(dp, do, drm, dro)
-        if kind == "bytes":
-            dbytes = hip_rt.DeviceBuffer(max(payload[0].nbytes, 1))
-            doffs = hip_rt.DeviceBuffer(payload[1].nbytes)
-        stream.synchronize()
-        t0 = time.perf_counter()
-        if kind == "u64":
-            ctx.build_u64_device(payload, n, cfg, outs, ds, stream)
-            t1 = t0
-        else:
-            ctx.copy_to_device(dbytes, payload[0], stream)
-            ctx.copy_to_device(doffs, payload[1], stream)
-            t1 = time.perf_counter()
-            ctx.build_bytes_device(dbytes, doffs, n, cfg, outs, ds, stream)
-        stream.synchronize()
-        build_s.append(time.perf_counter() - t0)
-        upload_s.append(t1 - t0)
+    def sequential_pass():
+        bs, us = [], []
+        for kind, n, payload, cfg in reqs:
+            np_, B = ph.sizes(cfg, n)
+            rc = ph.remap_capacity(cfg, n)
+            dp, do, ds = (hip_rt.DeviceBuffer(max(np_ * B, 1) * 4), hip_rt.DeviceBuffer((np_ + 1) * 4),
+                          hip_rt.DeviceBuffer(4))
+            drm = hip_rt.DeviceBuffer(max(rc, 1) * 4) if rc else None
+            dro = hip_rt.DeviceBuffer((np_ + 1) * 4) if rc else None
+            outs = ph.device_outputs(dp, do, drm, dro)
+            if kind == "bytes":
+                dbytes = hip_rt.DeviceBuffer(max(payload[0].nbytes, 1))
+                doffs = hip_rt.DeviceBuffer(payload[1].nbytes)
+            stream.synchronize()
+            t0 = time.perf_counter()
+            if kind == "u64":
+                ctx.build_u64_device(payload, n, cfg, outs, ds, stream)
+                t1 = t0
+            else:
+                ctx.copy_to_device(dbytes, payload[0], stream)
+                ctx.copy_to_device(doffs, payload[1], stream)
+                t1 = time.perf_counter()
+                ctx.build_bytes_device(dbytes, doffs, n, cfg, outs, ds, stream)
+            stream.synchronize()
+            bs.append(time.perf_counter() - t0)
+            us.append(t1 - t0)
+        return bs, us
+
+    # two passes; each request's faster one is the sequential service time
+    (b1, u1), (b2, u2) = sequential_pass(), sequential_pass()
+    build_s = [min(x, y) for x, y in zip(b1, b2)]
+    upload_s = [u if b == x else v for b, x, u, v in zip(build_s, b1, u1, u2)]
     done, lat = 0.0, []
     for a, b in zip(arrivals, build_s):
         done = max(done, a) + b
@@ -442,8 +453,15 @@ def run_serve(args) -> dict:
         return st, statuses
 
     serve(False)  # warm-up pass, untimed: the device pool grows to the trace's working set
-    st, statuses = serve(True)
-    burst, b_statuses = serve(False)  # every request at once: the server's capacity
+    # three passes each, paced at the arrivals and all at once (the server's
+    # capacity); the pass of median span is reported
+    def median_pass(paced: bool):
+        runs = [serve(paced) for _ in range(3)]
+        runs.sort(key=lambda r: r[0]["span_s"])
+        return runs[1][0], [x for r in runs for x in r[1]], [total / r[0]["span_s"] / 1e6 for r in runs]
+
+    st, statuses, paced_rates = median_pass(True)
+    burst, b_statuses, burst_rates = median_pass(False)
     lat_ms = np.array(lat) * 1e3
     arrival_span = float(arrivals[-1] - arrivals[0])
     return {
@@ -454,6 +472,7 @@ def run_serve(args) -> dict:
         "p50_ms": round(st["p50_ms"], 2),
         "p99_ms": round(st["p99_ms"], 2),
         "served_mkeys_s": round(total / st["span_s"] / 1e6, 2),
+        "served_mkeys_s_passes": [round(x, 1) for x in sorted(paced_rates)],
         "sequential_mkeys_s": round(total / seq_span / 1e6, 2),
         "speedup_vs_sequential": round(seq_span / st["span_s"], 2),
         "sequential_p50_ms": round(float(np.percentile(lat_ms, 50)), 2),
@@ -464,6 +483,7 @@ def run_serve(args) -> dict:
         "grouped_launches": st["grouped_launches"],
         "grouped_requests": st["grouped_requests"],
         "burst": {"served_mkeys_s": round(total / burst["span_s"] / 1e6, 2),
+                  "served_mkeys_s_passes": [round(x, 1) for x in sorted(burst_rates)],
                   "speedup_vs_sequential_builds": round(sum(build_s) / burst["span_s"], 2),
                   "p50_ms": round(burst["p50_ms"], 2), "p99_ms": round(burst["p99_ms"], 2),
                   "grouped_launches": burst["grouped_launches"], "grouped_requests": burst["grouped_requests"]},
