@@ -282,15 +282,17 @@ __device__ __forceinline__ uint32_t scan_pass(const uint32_t* Elane, uint32_t pk
     const uint32_t pka[1] = {pk};
     or_bucket<1, H>(eb, pka, gbase, s, acc);
   }
-  uint32_t fr[H];
-  uint64_t any[H], all = 0;
+  uint32_t fr[H], frall = 0;
 #pragma unroll
   for (int h = 0; h < H; h++) {
     fr[h] = ~acc[h] & vfull[h];
-    any[h] = __ballot(fr[h] != 0);
-    all |= any[h];
+    frall |= fr[h];
   }
-  if (!all) return ~0u;
+  // one vector compare decides the common case, no free offset at all
+  if (!__ballot(frall != 0)) return ~0u;
+  uint64_t any[H];
+#pragma unroll
+  for (int h = 0; h < H; h++) any[h] = __ballot(fr[h] != 0);
 #pragma unroll
   for (int h = 0; h < H; h++) {
     if (any[h]) {
