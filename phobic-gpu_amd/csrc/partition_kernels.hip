@@ -1090,11 +1090,11 @@ hipError_t launch_hash_route(int kind, int out_fmt, const HashRouteArgs& a, uint
   }
   // batched builds (sets) take u64 keys or byte strings
   if (a.sets && kind == kKindKey) return hipErrorInvalidValue;
-  if (kind == kKindU64 && !a.sets) {  // the pipelined kernel
+  if (kind == kKindU64 && !a.sets) {  // the pipelined kernel: raw keys, or final records when L = 1
     const uint32_t pp = pipe_per(threads, per, out_fmt, a.fanout, out_fmt != kFmtRaw, false);
     if (out_fmt == kFmtRaw) return route_keys_shape<kFmtRaw>(threads, pp, a, s);
-    if (out_fmt == kFmtRec) return route_keys_shape<kFmtRec>(threads, pp, a, s);
-    return route_keys_shape<kFmtFinal>(threads, pp, a, s);
+    if (out_fmt == kFmtFinal) return route_keys_shape<kFmtFinal>(threads, pp, a, s);
+    return hipErrorInvalidValue;
   }
   per = fitting_per(threads, per, out_fmt, a.fanout, out_fmt != kFmtRaw);
   switch (kind) {

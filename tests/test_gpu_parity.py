@@ -57,6 +57,26 @@ def test_u64_bit_exact_vs_oracle(ctx, n, lam, P, seed, kseed):
     assert_bijection(hk, r.pilots, r.offsets, lam, P)
 
 
+def test_split_with_more_parents_than_the_pipeline_takes(ctx):
+    """Fan-out 16 at 19,532 partitions (5M keys, P 256): four routing levels,
+    the last one splitting 1,221 parent bins, more than the pipelined split
+    kernel takes (1024), so it runs the per-tile split; the levels before it
+    run the pipelined kernels (raw keys to raw keys).  Bit-exact against the
+    oracle."""
+    keys = po.splitmix64(31, 5_000_000)
+    lam, P = 4.0, 256
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(**{**saved, "max_fanout": 16})
+        r = ctx.build_u64(keys, ph.make_config(lam, P))
+    finally:
+        ctx.set_tuning(**saved)
+    st, pil, off, hk = oracle_u64(keys, lam, P, 0)
+    assert st == 0 and r.np_ == 19532
+    assert np.array_equal(r.offsets, off)
+    assert np.array_equal(r.pilots, pil)
+
+
 def random_strings(n, lo, hi, seed, printable=True):
     rng = np.random.default_rng(seed)
     lens = rng.integers(lo, hi + 1, n)
