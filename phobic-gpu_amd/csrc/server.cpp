@@ -52,6 +52,7 @@ constexpr uint64_t kSmallKeys = 4000000;
 constexpr uint64_t kGroupKeys = 64000000;
 constexpr uint64_t kGroupKeysLarge = 256000000;
 constexpr uint32_t kGroupSets = 1024;  // requests of one shared launch at most
+constexpr size_t kGroupScan = 4096;    // queued requests a job's leader looks through for its group
 
 uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
 
@@ -192,7 +193,9 @@ std::shared_ptr<Job> form_job(phobic_server* s, std::deque<std::shared_ptr<Reque
     if (groupable(**it)) {
       uint64_t keys = (*it)->req.n;
       const uint64_t cap = large(**it) ? kGroupKeysLarge : kGroupKeys;
-      for (auto jt = std::next(it); jt != s->queue.end() && j->reqs.size() < kGroupSets; ++jt)
+      size_t scanned = 0;  // admission looks at a bounded part of a long queue
+      for (auto jt = std::next(it); jt != s->queue.end() && j->reqs.size() < kGroupSets && scanned < kGroupScan;
+           ++jt, ++scanned)
         if (groupable(**jt) && same_class(**it, **jt) && keys + (*jt)->req.n <= cap) {
           j->reqs.push_back(*jt);
           keys += (*jt)->req.n;
