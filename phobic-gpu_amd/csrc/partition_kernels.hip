@@ -1082,10 +1082,13 @@ hipError_t launch_hash_route(int kind, int out_fmt, const HashRouteArgs& a, uint
       !(threads == 512 || threads == 1024) || (uint64_t)a.G * a.cap > 0xFFFFFFFFull)
     return hipErrorInvalidValue;
   // A string tile waits on two dependent loads per key (offsets, then
-  // bytes): tiles of 4096 records hide that better than longer runs per bin
-  // pay (5.2 against 5.4 ms for 2e8 strings with 8192, 6.1 ms with 10,240).
+  // bytes), which sixteen waves of a 1024-thread workgroup hide better than
+  // eight: tiles of 1024 x 8 take 2e8 strings through level 1 in 4.5-4.6 ms
+  // against 5.9-6.0 ms for 512 x 8, 4.8-5.0 ms for 1024 x 4 or 512 x 4 and
+  // 5.9 ms for 512 x 12 (8192 records of 14 bytes, the bins and the
+  // fulcrums fit the LDS at every fan-out).
   if (kind == kKindBytes) {
-    threads = 512;
+    threads = 1024;
     per = 8;
   }
   // batched builds (sets) take u64 keys or byte strings
@@ -1103,11 +1106,11 @@ hipError_t launch_hash_route(int kind, int out_fmt, const HashRouteArgs& a, uint
       return hash_route_shape<kKindU64, kFmtFinal, true>(threads, per, a, s);
     case kKindBytes:
       if (a.sets) {
-        if (out_fmt == kFmtRec) return hash_route_go<kKindBytes, 512, 8, kFmtRec, true>(a, s);
-        return hash_route_go<kKindBytes, 512, 8, kFmtFinal, true>(a, s);
+        if (out_fmt == kFmtRec) return hash_route_go<kKindBytes, 1024, 8, kFmtRec, true>(a, s);
+        return hash_route_go<kKindBytes, 1024, 8, kFmtFinal, true>(a, s);
       }
-      if (out_fmt == kFmtRec) return hash_route_go<kKindBytes, 512, 8, kFmtRec, false>(a, s);
-      return hash_route_go<kKindBytes, 512, 8, kFmtFinal, false>(a, s);
+      if (out_fmt == kFmtRec) return hash_route_go<kKindBytes, 1024, 8, kFmtRec, false>(a, s);
+      return hash_route_go<kKindBytes, 1024, 8, kFmtFinal, false>(a, s);
     default:
       if (out_fmt == kFmtRec) return hash_route_shape<kKindKey, kFmtRec, false>(threads, per, a, s);
       return hash_route_shape<kKindKey, kFmtFinal, false>(threads, per, a, s);
