@@ -449,8 +449,10 @@ __global__ __launch_bounds__(kT) void k_split_level(SplitArgs a) {
 //   D  bin starts in the tile; clear the counts for the next tile     barrier
 //   E  bin bases and limits (the reserved ranges arrive here); scatter the
 //      current tile into LDS in bin order: it is now the staged tile  barrier
-// The records each child bin receives, and where, are exactly those of the
-// per-tile kernels: bins still fill in the order their ranges are reserved.
+// Each child bin receives the same records as with the per-tile kernels, in
+// the order its ranges are reserved (nondeterministic in both; no output
+// depends on it: the search orders a partition's keys by bucket, and a
+// bucket's pilot depends on the set of its keys only).
 //
 // LDS: lo[tile] u64 | meta[tile] | child[tile] u16 | hist[F] | lim[F] |
 //      base[F] | lstart[F] | wsum[16] | tstart[parents + 1] and
@@ -965,14 +967,32 @@ uint32_t pipe_per(uint32_t threads, uint32_t per, int out_fmt, uint32_t F, bool 
 }
 
 // A persistent launch: as many workgroups as the device holds at once (at
-// most `work`, at least one).
+// most `work`, at least one).  The device's answer for a (device, kernel,
+// threads, LDS) is kept per host thread, so repeated builds ask once.
 hipError_t persistent_grid(const void* fn, uint32_t threads, size_t lds, uint64_t work, uint32_t* grid) {
-  int dev = 0, cus = 0, per_cu = 0;
+  struct Entry {
+    const void* fn;
+    int dev;
+    uint32_t threads;
+    size_t lds;
+    uint64_t groups;
+  };
+  static thread_local Entry seen[16];
+  static thread_local unsigned next_slot = 0;
+  int dev = 0;
   hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds);
   if (e != hipSuccess) return e;
-  const uint64_t g = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1);
+  uint64_t g = 0;
+  for (const Entry& x : seen)
+    if (x.fn == fn && x.dev == dev && x.threads == threads && x.lds == lds) g = x.groups;
+  if (!g) {
+    int cus = 0, per_cu = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)threads, lds);
+    if (e != hipSuccess) return e;
+    g = (uint64_t)std::max(cus, 1) * (uint64_t)std::max(per_cu, 1);
+    seen[next_slot++ % 16] = Entry{fn, dev, threads, lds, g};
+  }
   *grid = (uint32_t)std::max<uint64_t>(1, std::min(g, work));
   return hipSuccess;
 }
