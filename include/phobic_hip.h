@@ -268,7 +268,7 @@ int phobic_server_destroy(phobic_server* server);
 /* ---- launch shapes and the autotuner ----
  * search_waves: partitions (one wave64 each) per search workgroup (1,2,4,8);
  * route_per: records per thread of a level-1 (hash and route) tile of u64
- *   or pre-hashed keys; byte strings always use tiles of 512 x 8;
+ *   or pre-hashed keys; byte strings always use tiles of 1024 x 8;
  * max_fanout: bins per partitioning level (16..1024);
  * search_mode: pilot-search variants, a mask of PHOBIC_SEARCH_* (0 = all
  *   of the fast paths);
@@ -278,7 +278,9 @@ int phobic_server_destroy(phobic_server* server);
  * route_threads: threads of a routing workgroup, 512 (0) or 1024; with 512
  *   route_per and split_per are 8, 12, 16, 20 or 24, with 1024 they are 4,
  *   6, 8, 10 or 12.  A level whose tile would not fit the workgroup's LDS
- *   uses the largest smaller per that does.
+ *   uses the largest smaller per that does.  Plain u64 builds at level 1,
+ *   and splits of at most 1024 parent bins, run persistent workgroups (one
+ *   per CU the device holds) over tiles of this shape.
  * Defaults {1, 12, 1024, 0, 12, 0, 1024}: tiles of 12,288 records, and two
  * levels of about sqrt(np) bins for a billion keys.
  * phobic_autotune times eight tile shapes x first_fanout {0, 48, 512} x
