@@ -273,6 +273,14 @@ struct BatchInfo {
   uint32_t* global_roff = nullptr;
   uint32_t* part_remap = nullptr;
   const uint32_t* set_remap_base = nullptr;
+  // sets of several configurations (null / 0: one configuration): per set
+  // its XXH3 seed, fulcrum table, buckets per partition and first pilot
+  const uint64_t* set_seed = nullptr;
+  const uint32_t* set_ftab = nullptr;
+  const uint32_t* ftabs = nullptr;
+  uint32_t ntabs = 0;
+  const uint32_t* set_B = nullptr;
+  const uint64_t* set_pilot_base = nullptr;
 };
 
 // Layout of the context scratch for a plan.
@@ -409,6 +417,10 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
   ha.sets = batch.sets;
   ha.set_key_off = batch.set_key_off;
   ha.set_part_base = batch.set_part_base;
+  ha.set_seed = batch.set_seed;
+  ha.set_ftab = batch.set_ftab;
+  ha.ftabs = batch.ftabs;
+  ha.ntabs = batch.ntabs;
   const uint32_t threads = ctx->tuning.route_threads ? ctx->tuning.route_threads : 512;
   TRY_HIP(launch_hash_route(kind, fmt(1), ha, threads, per, s));
   if (timed) TRY_HIP(hipEventRecord(ctx->ev[1], s));
@@ -472,6 +484,8 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
   ra.status = d_status;
   ra.sets = batch.sets;
   ra.set_part_base = batch.set_part_base;
+  ra.set_B = batch.set_B;
+  ra.set_pilot_base = batch.set_pilot_base;
   ra.mode = ctx->tuning.search_mode;
   TRY_HIP(launch_partition_search(ra, search_waves(ctx, pl), s));
   if (timed) {
@@ -694,9 +708,11 @@ uint64_t grain(uint64_t b) { return (b + kPoolGrain - 1) / kPoolGrain * kPoolGra
 
 // Byte offsets of a group build's tables inside ctx->grp.
 struct GroupTables {
-  uint64_t key_off, part_base, remap_base, goff, groff, part_remap, bytes;
+  uint64_t key_off, part_base, remap_base, goff, groff, part_remap;
+  uint64_t seed, setB, ftab, pilot_base, ftabs;  // sets of several configurations
+  uint64_t bytes;
 };
-GroupTables group_tables(uint32_t sets, uint32_t np) {
+GroupTables group_tables(uint32_t sets, uint32_t np, uint32_t ntabs = 0) {
   GroupTables t;
   uint64_t at = 0;
   auto take = [&at](uint64_t b) {
@@ -710,6 +726,12 @@ GroupTables group_tables(uint32_t sets, uint32_t np) {
   t.goff = take(4ull * (np + 1));
   t.groff = take(4ull * (np + 1));
   t.part_remap = take(4ull * (np + 1));
+  const bool mixed = ntabs != 0;
+  t.seed = take(mixed ? 8ull * sets : 0);
+  t.setB = take(mixed ? 4ull * sets : 0);
+  t.ftab = take(mixed ? 4ull * sets : 0);
+  t.pilot_base = take(mixed ? 8ull * (sets + 1) : 0);
+  t.ftabs = take(4ull * kFulcsTable * ntabs);
   t.bytes = at;
   return t;
 }
@@ -720,6 +742,7 @@ int group_plan(const phobic_config* cfg, const phobic::GroupLayout& gl, Plan* pl
   if (gl.n > 0xFFFFFFFFull || gl.np > (1u << 21)) return PHOBIC_E_INVALID_CONFIG;
   pl->n = gl.n;
   pl->np = gl.np;
+  if (gl.B) pl->B = gl.B;  // the largest of the sets' (the search's LDS layout)
   if (!plan_levels(pl)) return PHOBIC_E_INVALID_CONFIG;
   return PHOBIC_OK;
 }
@@ -754,33 +777,82 @@ uint64_t growth(const phobic_ctx* ctx, const Plan& pl, uint64_t grp_bytes, bool 
 
 }  // namespace
 
-int phobic::group_layout(const phobic_config* cfg, const uint64_t* set_sizes, uint32_t sets, GroupLayout* gl) {
-  if (!cfg || !gl || (sets && !set_sizes)) return PHOBIC_E_INVALID_CONFIG;
+int phobic::group_layout(const phobic_config* const* cfgs, const uint64_t* set_sizes, uint32_t sets,
+                         GroupLayout* gl) {
+  if (!cfgs || !gl || (sets && (!set_sizes || !cfgs[0]))) return PHOBIC_E_INVALID_CONFIG;
+  const phobic_config* c0 = sets ? cfgs[0] : nullptr;
+  if (!c0) return PHOBIC_E_INVALID_CONFIG;
   Plan one;
-  TRY(validate(cfg, 0, &one));
+  TRY(validate(c0, 0, &one));
+  auto alpha_of = [](const phobic_config* c) { return c->alpha == 0.0 ? 1.0 : c->alpha; };
+  bool mixed = false;
+  for (uint32_t s = 0; s < sets; s++) {
+    const phobic_config* c = cfgs[s];
+    if (!c || c->partition_size != c0->partition_size || alpha_of(c) != alpha_of(c0) ||
+        c->buckets != c0->buckets || c->fulcrums != c0->fulcrums)
+      return PHOBIC_E_INVALID_CONFIG;
+    if (c->lambda != c0->lambda || c->seed != c0->seed) mixed = true;
+  }
+  if (mixed && c0->fulcrums) return PHOBIC_E_INVALID_CONFIG;  // one custom table for all
   try {
     gl->key_off.assign(sets + 1, 0);
     gl->part_base.assign(sets + 1, 0);
     gl->remap_base.assign(sets + 1, 0);
+    gl->pilot_base.assign(sets + 1, 0);
+    gl->seed.clear();
+    gl->setB.clear();
+    gl->ftab.clear();
+    gl->ftabs.clear();
+    if (mixed) {
+      gl->seed.resize(sets);
+      gl->setB.resize(sets);
+      gl->ftab.resize(sets);
+    }
   } catch (...) {
     return PHOBIC_E_OOM;
   }
   gl->sets = sets;
-  gl->B = one.B;
-  uint64_t parts = 0, rem = 0;
+  gl->mixed = mixed;
+  gl->ntabs = 0;
+  uint32_t Bmax = 0;
+  std::vector<double> lambdas;  // of the fulcrum tables, in table order
+  uint64_t parts = 0, rem = 0, pil = 0;
   for (uint32_t s = 0; s < sets; s++) {
     Plan ps;
-    TRY(validate(cfg, set_sizes[s], &ps));
+    TRY(validate(cfgs[s], set_sizes[s], &ps));
     gl->key_off[s + 1] = gl->key_off[s] + set_sizes[s];
     parts += ps.np;
     rem += remap_capacity(ps);
+    pil += (uint64_t)ps.np * ps.B;
     if (parts > (1u << 21) || rem > 0xFFFFFFFFull || gl->key_off[s + 1] > 0xFFFFFFFFull)
       return PHOBIC_E_INVALID_CONFIG;
     gl->part_base[s + 1] = (uint32_t)parts;
     gl->remap_base[s + 1] = (uint32_t)rem;
+    gl->pilot_base[s + 1] = pil;
+    Bmax = std::max(Bmax, ps.B);
+    if (mixed) {
+      uint32_t t = 0;
+      while (t < lambdas.size() && lambdas[t] != cfgs[s]->lambda) t++;
+      if (t == lambdas.size()) {
+        if (t == kMaxFulcTabs) return PHOBIC_E_INVALID_CONFIG;
+        uint32_t f[kFulcsTable];
+        config_fulcrums(cfgs[s], f);
+        try {
+          lambdas.push_back(cfgs[s]->lambda);
+          gl->ftabs.insert(gl->ftabs.end(), f, f + kFulcsTable);
+        } catch (...) {
+          return PHOBIC_E_OOM;
+        }
+      }
+      gl->seed[s] = cfgs[s]->seed;
+      gl->setB[s] = ps.B;
+      gl->ftab[s] = t;
+    }
   }
+  gl->ntabs = (uint32_t)lambdas.size();
   gl->np = (uint32_t)parts;
   gl->n = gl->key_off[sets];
+  gl->B = Bmax;
   gl->a_ppm = one.a_ppm;
   return PHOBIC_OK;
 }
@@ -789,7 +861,7 @@ uint64_t phobic::group_scratch_growth(const phobic_ctx* ctx, const phobic_config
                                       bool from_empty) {
   Plan pl;
   if (group_plan(cfg, gl, &pl) != PHOBIC_OK) return 0;
-  return growth(ctx, pl, group_tables(gl.sets, gl.np).bytes, from_empty);
+  return growth(ctx, pl, group_tables(gl.sets, gl.np, gl.ntabs).bytes, from_empty);
 }
 
 uint64_t phobic::build_scratch_growth(const phobic_ctx* ctx, const phobic_config* cfg, uint64_t n, bool from_empty) {
@@ -891,13 +963,25 @@ int phobic::group_build(phobic_ctx* ctx, const phobic_config* cfg, int kind, con
   const bool remap = pl.a_ppm < 1000000u;
   if (!block.pilots || !block.offsets || (remap && (!block.remap || !block.remap_offsets))) return PHOBIC_E_INVALID_CONFIG;
   TRY(set_device(ctx));
-  const GroupTables t = group_tables(gl.sets, gl.np);
+  const GroupTables t = group_tables(gl.sets, gl.np, gl.ntabs);
   TRY_HIP(ctx->grp.ensure(t.bytes, s));
   char* g = ctx->grp.as<char>();
   // from pageable memory: each copy has read its source when it returns
   TRY_HIP(hipMemcpyAsync(g + t.key_off, gl.key_off.data(), 8ull * (gl.sets + 1), hipMemcpyHostToDevice, s));
   TRY_HIP(hipMemcpyAsync(g + t.part_base, gl.part_base.data(), 4ull * (gl.sets + 1), hipMemcpyHostToDevice, s));
   TRY_HIP(hipMemcpyAsync(g + t.remap_base, gl.remap_base.data(), 4ull * (gl.sets + 1), hipMemcpyHostToDevice, s));
+  if (gl.mixed) {
+    if (gl.seed.size() != gl.sets || gl.setB.size() != gl.sets || gl.ftab.size() != gl.sets ||
+        gl.pilot_base.size() != gl.sets + 1 || gl.ftabs.size() != (size_t)gl.ntabs * kFulcsTable || !gl.ntabs)
+      return PHOBIC_E_INVALID_CONFIG;
+    for (uint32_t x = 0; x < gl.sets; x++)
+      if (gl.ftab[x] >= gl.ntabs || gl.setB[x] > gl.B) return PHOBIC_E_INVALID_CONFIG;
+    TRY_HIP(hipMemcpyAsync(g + t.seed, gl.seed.data(), 8ull * gl.sets, hipMemcpyHostToDevice, s));
+    TRY_HIP(hipMemcpyAsync(g + t.setB, gl.setB.data(), 4ull * gl.sets, hipMemcpyHostToDevice, s));
+    TRY_HIP(hipMemcpyAsync(g + t.ftab, gl.ftab.data(), 4ull * gl.sets, hipMemcpyHostToDevice, s));
+    TRY_HIP(hipMemcpyAsync(g + t.pilot_base, gl.pilot_base.data(), 8ull * (gl.sets + 1), hipMemcpyHostToDevice, s));
+    TRY_HIP(hipMemcpyAsync(g + t.ftabs, gl.ftabs.data(), 4ull * gl.ftabs.size(), hipMemcpyHostToDevice, s));
+  }
   TRY(prepare_build(ctx, cfg, pl, s));
   BatchInfo bi;
   bi.sets = gl.sets;
@@ -907,6 +991,14 @@ int phobic::group_build(phobic_ctx* ctx, const phobic_config* cfg, int kind, con
   bi.global_offsets = reinterpret_cast<uint32_t*>(g + t.goff);
   bi.global_roff = reinterpret_cast<uint32_t*>(g + t.groff);
   bi.part_remap = reinterpret_cast<uint32_t*>(g + t.part_remap);
+  if (gl.mixed) {
+    bi.set_seed = reinterpret_cast<const uint64_t*>(g + t.seed);
+    bi.set_B = reinterpret_cast<const uint32_t*>(g + t.setB);
+    bi.set_ftab = reinterpret_cast<const uint32_t*>(g + t.ftab);
+    bi.set_pilot_base = reinterpret_cast<const uint64_t*>(g + t.pilot_base);
+    bi.ftabs = reinterpret_cast<const uint32_t*>(g + t.ftabs);
+    bi.ntabs = gl.ntabs;
+  }
   KeySource src{};
   if (kind == kKindU64) {
     src.u64 = static_cast<const uint64_t*>(d_keys);

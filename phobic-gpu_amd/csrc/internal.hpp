@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/phobic_hip.h"
+#include "kernels.hpp"
 
 namespace phobic {
 
@@ -13,23 +14,36 @@ namespace phobic {
 hipStream_t ctx_stream(phobic_ctx* ctx);
 int ctx_device(const phobic_ctx* ctx);
 
-// A group: several independent key sets of one configuration built as one
-// flattened launch (every set's outputs equal building it alone).  Set s
-// holds keys [key_off[s], key_off[s+1]) of the group's key array; its
-// outputs in the group's output block are
-//   pilots         block.pilots + B * part_base[s]     (np_s * B)
+// A group: several independent key sets built as one flattened launch
+// (every set's outputs equal building it alone with its own
+// configuration).  The sets share the partition size, the load factor and
+// the bucketer (a custom fulcrum table only when every set has the same
+// configuration); lambda and the XXH3 seed may differ from set to set, with
+// at most kMaxFulcTabs distinct lambdas.  Set s holds keys
+// [key_off[s], key_off[s+1]) of the group's key array; its outputs in the
+// group's output block are
+//   pilots         block.pilots + pilot_base[s]        (np_s * B_s)
 //   offsets        block.offsets + part_base[s] + s    (np_s + 1)
 //   remap_offsets  block.remap_offsets + part_base[s] + s (np_s + 1, alpha < 1)
 //   remap          block.remap + remap_base[s]          (alpha < 1)
-// with block.pilots holding B * np entries, block.offsets and
+// with block.pilots holding pilot_base[sets] entries, block.offsets and
 // block.remap_offsets np + sets, block.remap remap_base[sets].
 struct GroupLayout {
-  uint32_t sets = 0, np = 0, B = 0, a_ppm = 1000000;
+  uint32_t sets = 0, np = 0, B = 0, a_ppm = 1000000;  // B: the largest of the sets'
   uint64_t n = 0;
   std::vector<uint64_t> key_off;
   std::vector<uint32_t> part_base, remap_base;
+  // per set: its pilots start at pilot_base[s] (sets + 1 entries); when the
+  // sets' configurations differ (mixed), its seed, buckets per partition and
+  // fulcrum table (ntabs tables of kFulcsTable entries in ftabs)
+  std::vector<uint64_t> pilot_base;
+  bool mixed = false;
+  std::vector<uint64_t> seed;
+  std::vector<uint32_t> setB, ftab, ftabs;
+  uint32_t ntabs = 0;
 };
-int group_layout(const phobic_config* cfg, const uint64_t* set_sizes, uint32_t sets, GroupLayout* gl);
+// cfgs[s]: the configuration of set s
+int group_layout(const phobic_config* const* cfgs, const uint64_t* set_sizes, uint32_t sets, GroupLayout* gl);
 // Enqueues the group's build on s (kind: u64 keys, or bytes with d_offs the
 // group's n + 1 string offsets into d_keys).
 int group_build(phobic_ctx* ctx, const phobic_config* cfg, int kind, const void* d_keys, const uint64_t* d_offs,

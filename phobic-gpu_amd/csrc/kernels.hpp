@@ -55,7 +55,17 @@ struct HashRouteArgs {
   uint32_t sets;
   const uint64_t* set_key_off;
   const uint32_t* set_part_base;
+  // sets of several configurations (null: every set uses seed and fulcs):
+  // set s hashes with set_seed[s] and maps buckets with fulcrum table
+  // set_ftab[s] of the ntabs tables in ftabs (kFulcsTable entries each)
+  const uint64_t* set_seed;
+  const uint32_t* set_ftab;
+  const uint32_t* ftabs;
+  uint32_t ntabs;
 };
+
+// Most fulcrum tables one launch holds (sets that differ in lambda).
+constexpr uint32_t kMaxFulcTabs = 4;
 
 // Levels 2..L: split every parent bin into `fanout` child bins.
 struct SplitArgs {
@@ -98,9 +108,14 @@ struct SearchArgs {
   uint32_t* status;
   // batched builds (sets > 0): global partition p belongs to the set s with
   // set_part_base[s] <= p < set_part_base[s+1]; its pilots go to
-  // pilots + B*set_part_base[s], interleaved by that set's partition count
+  // pilots + B*set_part_base[s], interleaved by that set's partition count.
+  // Sets of several configurations: set s has set_B[s] <= B buckets per
+  // partition and its pilots start at pilots + set_pilot_base[s] (null: B
+  // and B*set_part_base[s]); B sizes the LDS layout.
   uint32_t sets;
   const uint32_t* set_part_base;
+  const uint32_t* set_B;
+  const uint64_t* set_pilot_base;
   uint32_t mode;  // mask of kSearchSinglePilot / kSearchNarrow
 };
 

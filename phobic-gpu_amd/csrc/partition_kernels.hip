@@ -60,11 +60,12 @@ struct Meta<kFmtFinal> {
 // Dynamic LDS of a routing tile of `tile` records into F bins (byte offsets):
 //   lo[tile] u64 | meta[tile] (4, 2 or 0 bytes) | child[tile] u16 |
 //   base[F] u32 | hist[F] u32 | lstart[F] u32 | wsum[kMaxWaves] u32 |
-//   fulcrums (2048 u32) when the level maps keys to buckets
+//   `tabs` fulcrum tables (2048 u32 each) when the level maps keys to
+//   buckets (one, or one per lambda of a batch of several configurations)
 struct TileLayout {
   uint32_t lo, meta, child, base, hist, lstart, wsum, fulcs, bytes;
 };
-__host__ __device__ inline TileLayout tile_layout(uint32_t tile, int fmt, uint32_t F, bool fulcs) {
+__host__ __device__ inline TileLayout tile_layout(uint32_t tile, int fmt, uint32_t F, uint32_t tabs) {
   TileLayout l;
   uint32_t at = 0;
   auto take = [&at](uint32_t bytes, uint32_t align) {
@@ -80,7 +81,7 @@ __host__ __device__ inline TileLayout tile_layout(uint32_t tile, int fmt, uint32
   l.hist = take(F * 4, 4);
   l.lstart = take(F * 4, 4);
   l.wsum = take(kMaxWaves * 4, 4);
-  l.fulcs = take(fulcs ? kFulcs * 4 : 0, 4);
+  l.fulcs = take(tabs * kFulcs * 4, 4);
   l.bytes = at;
   return l;
 }
@@ -268,10 +269,13 @@ __global__ __launch_bounds__(kT) void k_hash_route(HashRouteArgs a) {
   constexpr bool kBuckets = OUT != kFmtRaw;
   static_assert(OUT != kFmtRaw || (KIND == kKindU64 && !SETS), "only u64 keys of plain builds travel raw");
   extern __shared__ __align__(16) uint8_t lds[];
-  const TileLayout lay = tile_layout(kTile, OUT, a.fanout, kBuckets);
+  // sets of several configurations bring a fulcrum table per lambda
+  const uint32_t tabs = kBuckets ? (SETS && a.ntabs ? a.ntabs : 1u) : 0u;
+  const TileLayout lay = tile_layout(kTile, OUT, a.fanout, tabs);
   const Tile<OUT> s = tile_at<OUT>(lds, lay);
   if constexpr (kBuckets) {
-    for (uint32_t i = threadIdx.x; i < kFulcs; i += kT) s.fulcs[i] = a.fulcs[i];
+    const uint32_t* tsrc = SETS && a.ntabs ? a.ftabs : a.fulcs;
+    for (uint32_t i = threadIdx.x; i < tabs * kFulcs; i += kT) s.fulcs[i] = tsrc[i];
     __syncthreads();
   }
   const uint64_t ntiles = (a.n + kTile - 1) / kTile;
@@ -291,6 +295,8 @@ __global__ __launch_bounds__(kT) void k_hash_route(HashRouteArgs a) {
     uint32_t set = 0;
     uint64_t set_end = ~0ull;
     uint32_t pb = 0, pn = a.np;
+    uint64_t seed = a.seed;  // of the current set
+    uint32_t ftab = 0;       // its fulcrum table, as an offset into s.fulcs
     if constexpr (SETS) {
       // set of this thread's first key: last s with set_key_off[s] <= i
       const uint64_t i0 = t * kTile + threadIdx.x;
@@ -305,6 +311,10 @@ __global__ __launch_bounds__(kT) void k_hash_route(HashRouteArgs a) {
       if (set + 1 < a.sets) set_end = a.set_key_off[set + 1];
       pb = a.set_part_base[set];
       pn = a.set_part_base[set + 1] - pb;
+      if (a.set_seed) {
+        seed = a.set_seed[set];
+        ftab = a.set_ftab[set] * kFulcs;
+      }
     }
     // issued after every other load this tile waits for: vector memory
     // counters retire in order, so a wait on a later load would wait for
@@ -322,7 +332,6 @@ __global__ __launch_bounds__(kT) void k_hash_route(HashRouteArgs a) {
       meta[k] = 0;
       child[k] = 0;
       if (valid[k]) {
-        const U128 h = KIND == kKindU64 ? xxh3_128_u64(kv[k], a.seed) : hash_key<KIND>(a.src, i, a.seed);
         if (SETS && i >= set_end) {  // a later set (loads only at a boundary)
           do {
             set++;
@@ -330,7 +339,12 @@ __global__ __launch_bounds__(kT) void k_hash_route(HashRouteArgs a) {
           } while (i >= set_end);
           pb = a.set_part_base[set];
           pn = a.set_part_base[set + 1] - pb;
+          if (a.set_seed) {
+            seed = a.set_seed[set];
+            ftab = a.set_ftab[set] * kFulcs;
+          }
         }
+        const U128 h = KIND == kKindU64 ? xxh3_128_u64(kv[k], seed) : hash_key<KIND>(a.src, i, seed);
         const uint32_t p = pb + partition_of((uint32_t)h.lo, pn);
         // level-1 bin c = p / G and r = p mod G: the f32 quotient estimate
         // is within one of c (c < 2^11, relative error below 2^-22), G <
@@ -348,7 +362,7 @@ __global__ __launch_bounds__(kT) void k_hash_route(HashRouteArgs a) {
           lo[k] = kv[k];
         } else {
           lo[k] = h.hi;
-          const uint32_t b = bucket_of((uint32_t)(h.lo >> 32), s.fulcs);
+          const uint32_t b = bucket_of((uint32_t)(h.lo >> 32), s.fulcs + ftab);
           meta[k] = OUT == kFmtRec ? ((uint32_t)r << 16) | b : b;
         }
         child[k] = min(c, a.fanout - 1);
@@ -909,8 +923,15 @@ __global__ __launch_bounds__(kT) void k_split_pipe(SplitArgs a, uint32_t parents
 
 namespace {
 
-size_t route_lds(uint32_t tile, int out_fmt, uint32_t F, bool fulcs) {
-  return tile_layout(tile, out_fmt, F, fulcs).bytes;
+// Fulcrum tables a level-1 tile holds: none for raw keys, one, or one per
+// lambda of a batch of several configurations.
+uint32_t fulc_tabs(int out_fmt, bool sets, const HashRouteArgs& a) {
+  return out_fmt == kFmtRaw ? 0u : sets && a.ntabs ? a.ntabs : 1u;
+}
+
+// tabs: fulcrum tables the tile holds (0 when the level maps no buckets)
+size_t route_lds(uint32_t tile, int out_fmt, uint32_t F, uint32_t tabs) {
+  return tile_layout(tile, out_fmt, F, tabs).bytes;
 }
 
 // Tile shapes (threads, records per thread) a routing launch can take.
@@ -921,9 +942,9 @@ bool valid_shape(uint32_t threads, uint32_t per) {
 
 // The largest records per thread, at most `per`, whose tile of `threads`
 // threads fits the workgroup's LDS.
-uint32_t fitting_per(uint32_t threads, uint32_t per, int out_fmt, uint32_t F, bool fulcs) {
+uint32_t fitting_per(uint32_t threads, uint32_t per, int out_fmt, uint32_t F, uint32_t tabs) {
   for (uint32_t p : {24u, 20u, 16u, 12u, 10u, 8u, 6u, 4u})
-    if (p <= per && valid_shape(threads, p) && route_lds(threads * p, out_fmt, F, fulcs) <= kLdsLimit) return p;
+    if (p <= per && valid_shape(threads, p) && route_lds(threads * p, out_fmt, F, tabs) <= kLdsLimit) return p;
   return threads == 512 ? 8 : 4;
 }
 
@@ -932,7 +953,7 @@ hipError_t hash_route_go(const HashRouteArgs& a, hipStream_t s) {
   constexpr uint32_t kTile = kT * kPer;
   const uint64_t ntiles = (a.n + kTile - 1) / kTile;
   const uint32_t grid = (uint32_t)(ntiles < 4096 ? ntiles : 4096);
-  const size_t lds = route_lds(kTile, OUT, a.fanout, OUT != kFmtRaw);
+  const size_t lds = route_lds(kTile, OUT, a.fanout, fulc_tabs(OUT, SETS, a));
   hipLaunchKernelGGL((k_hash_route<KIND, kT, kPer, OUT, SETS>), dim3(grid), dim3(kT), lds, s, a);
   return hipGetLastError();
 }
@@ -1064,7 +1085,7 @@ hipError_t split_pipe_shape(uint32_t threads, uint32_t per, const SplitArgs& a, 
 
 template <uint32_t kT, uint32_t kPer, int IN, int OUT>
 hipError_t split_go(const SplitArgs& a, uint32_t parents, hipStream_t s) {
-  const size_t lds = route_lds(kT * kPer, OUT, a.fanout, IN == kFmtRaw && OUT == kFmtFinal);
+  const size_t lds = route_lds(kT * kPer, OUT, a.fanout, IN == kFmtRaw && OUT == kFmtFinal ? 1u : 0u);
   hipLaunchKernelGGL((k_split_level<kT, kPer, IN, OUT>), dim3(parents * a.slices), dim3(kT), lds, s, a);
   return hipGetLastError();
 }
@@ -1092,7 +1113,7 @@ hipError_t split_shape(uint32_t threads, uint32_t per, const SplitArgs& a, uint3
 }  // namespace
 
 uint32_t split_tile_per(uint32_t threads, uint32_t per, int in_fmt, int out_fmt, uint32_t fanout) {
-  return fitting_per(threads, per, out_fmt, fanout, in_fmt == kFmtRaw && out_fmt == kFmtFinal);
+  return fitting_per(threads, per, out_fmt, fanout, in_fmt == kFmtRaw && out_fmt == kFmtFinal ? 1u : 0u);
 }
 
 hipError_t launch_hash_route(int kind, int out_fmt, const HashRouteArgs& a, uint32_t threads, uint32_t per,
@@ -1111,20 +1132,27 @@ hipError_t launch_hash_route(int kind, int out_fmt, const HashRouteArgs& a, uint
     threads = 1024;
     per = 8;
   }
-  // batched builds (sets) take u64 keys or byte strings
+  // batched builds (sets) take u64 keys or byte strings; sets of several
+  // configurations bring a seed and a fulcrum table index per set
   if (a.sets && kind == kKindKey) return hipErrorInvalidValue;
+  if (a.ntabs > kMaxFulcTabs || (a.set_seed != nullptr) != (a.ntabs != 0) ||
+      (a.ntabs && (!a.sets || !a.set_ftab || !a.ftabs)))
+    return hipErrorInvalidValue;
   if (kind == kKindU64 && !a.sets) {  // the pipelined kernel: raw keys, or final records when L = 1
     const uint32_t pp = pipe_per(threads, per, out_fmt, a.fanout, out_fmt != kFmtRaw, false);
     if (out_fmt == kFmtRaw) return route_keys_shape<kFmtRaw>(threads, pp, a, s);
     if (out_fmt == kFmtFinal) return route_keys_shape<kFmtFinal>(threads, pp, a, s);
     return hipErrorInvalidValue;
   }
-  per = fitting_per(threads, per, out_fmt, a.fanout, out_fmt != kFmtRaw);
+  per = fitting_per(threads, per, out_fmt, a.fanout, fulc_tabs(out_fmt, a.sets != 0, a));
   switch (kind) {
     case kKindU64:  // batched (plain builds took the pipelined kernel)
       if (out_fmt == kFmtRec) return hash_route_shape<kKindU64, kFmtRec, true>(threads, per, a, s);
       return hash_route_shape<kKindU64, kFmtFinal, true>(threads, per, a, s);
     case kKindBytes:
+      // 8192 Rec records, 1024 bins and four fulcrum tables take 156 KiB
+      if (route_lds(1024 * 8, out_fmt, a.fanout, fulc_tabs(out_fmt, a.sets != 0, a)) > kLdsLimit)
+        return hipErrorInvalidValue;
       if (a.sets) {
         if (out_fmt == kFmtRec) return hash_route_go<kKindBytes, 1024, 8, kFmtRec, true>(a, s);
         return hash_route_go<kKindBytes, 1024, 8, kFmtFinal, true>(a, s);

@@ -360,13 +360,16 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // wave-uniform values are made visibly uniform so that they live in
   // SGPRs and loops over them stay scalar
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t cap = a.cap, B = a.B, tcap = a.tcap;
+  const uint32_t cap = a.cap, tcap = a.tcap;
+  // B: buckets per partition (of this partition's set, below); a.B sizes the
+  // LDS layout and is the largest of them
+  uint32_t B = a.B;
   const uint32_t EW = e_words(tcap), CW = (tcap + 31) / 32 + 1;
   // the offsets of search_layout (layout_matches_kernel checks them on the
   // host before every launch)
-  uint32_t* rt = smem + wave * lds_words(tcap, B);  // 1 word per rank
-  uint32_t* cp = rt + B;   // counts, then pilots, per bucket index
-  uint32_t* U = cp + B;
+  uint32_t* rt = smem + wave * lds_words(tcap, a.B);  // 1 word per rank
+  uint32_t* cp = rt + a.B;  // counts, then pilots, per bucket index
+  uint32_t* U = cp + a.B;
   uint32_t* hist = U;          // steps 1-2
   uint32_t* bpos = U;          // steps 3-4
   uint32_t* E = U;             // steps 5-6
@@ -393,7 +396,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     // skip empty sets that share the same base
     while (lo_s + 1 < a.sets && a.set_part_base[lo_s + 1] <= p) lo_s++;
     const uint32_t pb = a.set_part_base[lo_s];
-    out_pilots = a.pilots + (uint64_t)B * pb;
+    if (a.set_B) {  // sets of several configurations
+      B = __builtin_amdgcn_readfirstlane(a.set_B[lo_s]);
+      out_pilots = a.pilots + a.set_pilot_base[lo_s];
+    } else {
+      out_pilots = a.pilots + (uint64_t)B * pb;
+    }
     out_p = p - pb;
     out_np = a.set_part_base[lo_s + 1] - pb;
   }
@@ -922,6 +930,7 @@ size_t search_lds_bytes(uint32_t tcap, uint32_t B, uint32_t waves) { return (siz
 hipError_t launch_partition_search(const SearchArgs& a, uint32_t waves, hipStream_t s) {
   if (a.np == 0) return hipSuccess;
   if (!layout_matches_kernel(a.tcap, a.B)) return hipErrorInvalidValue;
+  if ((a.set_B != nullptr) != (a.set_pilot_base != nullptr) || (a.set_B && !a.sets)) return hipErrorInvalidValue;
   const uint32_t groups = (a.np + waves - 1) / waves;
   const size_t lds = search_lds_bytes(a.tcap, a.B, waves);
   switch (waves) {

@@ -106,7 +106,7 @@ int upload(phobic_ctx* ctx, void* dst, const void* src, uint64_t bytes, hipStrea
 uint64_t group_out_bytes(const phobic::GroupLayout& gl) {
   const uint64_t entries = (uint64_t)gl.np + gl.sets;
   const bool remap = gl.a_ppm < 1000000u;
-  return align256((uint64_t)gl.np * gl.B * 4) + align256(entries * 4) +
+  return align256(gl.pilot_base[gl.sets] * 4) + align256(entries * 4) +
          (remap ? align256((uint64_t)gl.remap_base[gl.sets] * 4) + align256(entries * 4) : 0) + 256;
 }
 
@@ -134,10 +134,11 @@ bool large(const Request& r) { return r.req.n > kSmallKeys; }
 
 double alpha_of(const phobic_config& c) { return c.alpha == 0.0 ? 1.0 : c.alpha; }
 
+// Requests one launch can build together: lambda and the seed may differ
+// (at most kMaxFulcTabs lambdas in a group, counted in form_job).
 bool same_class(const Request& a, const Request& b) {
-  return a.req.kind == b.req.kind && a.cfg.lambda == b.cfg.lambda && a.cfg.partition_size == b.cfg.partition_size &&
-         a.cfg.buckets == b.cfg.buckets && a.cfg.seed == b.cfg.seed && alpha_of(a.cfg) == alpha_of(b.cfg) &&
-         large(a) == large(b);
+  return a.req.kind == b.req.kind && a.cfg.partition_size == b.cfg.partition_size && a.cfg.buckets == b.cfg.buckets &&
+         alpha_of(a.cfg) == alpha_of(b.cfg) && large(a) == large(b);
 }
 
 }  // namespace
@@ -194,22 +195,31 @@ std::shared_ptr<Job> form_job(phobic_server* s, std::deque<std::shared_ptr<Reque
       uint64_t keys = (*it)->req.n;
       const uint64_t cap = large(**it) ? kGroupKeysLarge : kGroupKeys;
       size_t scanned = 0;  // admission looks at a bounded part of a long queue
+      double lambdas[phobic::kMaxFulcTabs] = {(*it)->cfg.lambda};
+      uint32_t nl = 1;
       for (auto jt = std::next(it); jt != s->queue.end() && j->reqs.size() < kGroupSets && scanned < kGroupScan;
-           ++jt, ++scanned)
-        if (groupable(**jt) && same_class(**it, **jt) && keys + (*jt)->req.n <= cap) {
-          j->reqs.push_back(*jt);
-          keys += (*jt)->req.n;
-        }
+           ++jt, ++scanned) {
+        if (!groupable(**jt) || !same_class(**it, **jt) || keys + (*jt)->req.n > cap) continue;
+        uint32_t l = 0;
+        while (l < nl && lambdas[l] != (*jt)->cfg.lambda) l++;
+        if (l == phobic::kMaxFulcTabs) continue;  // a lambda more than one launch holds
+        if (l == nl) lambdas[nl++] = (*jt)->cfg.lambda;
+        j->reqs.push_back(*jt);
+        keys += (*jt)->req.n;
+      }
     }
     if (j->reqs.size() > 1) {
       std::vector<uint64_t> sizes;
+      std::vector<const phobic_config*> cfgs;
       sizes.reserve(j->reqs.size());
+      cfgs.reserve(j->reqs.size());
       uint64_t in = 0;
       for (const auto& r : j->reqs) {
         sizes.push_back(r->req.n);
+        cfgs.push_back(&r->cfg);
         in += r->req.kind == PHOBIC_KEYS_BYTES ? r->req.bytes_len : r->req.n * 8;
       }
-      if (phobic::group_layout(&j->reqs[0]->cfg, sizes.data(), (uint32_t)sizes.size(), &j->gl) != PHOBIC_OK) {
+      if (phobic::group_layout(cfgs.data(), sizes.data(), (uint32_t)sizes.size(), &j->gl) != PHOBIC_OK) {
         j->reqs.resize(1);  // build it alone
       } else {
         j->group = true;
@@ -421,7 +431,7 @@ int run_group(phobic_ctx* ctx, Job& j, hipStream_t stream) {
   const uint64_t entries = (uint64_t)gl.np + gl.sets;
   phobic_outputs bo{};
   bo.pilots = reinterpret_cast<uint32_t*>(p);
-  p += align256((uint64_t)gl.np * gl.B * 4);
+  p += align256(gl.pilot_base[gl.sets] * 4);
   bo.offsets = reinterpret_cast<uint32_t*>(p);
   p += align256(entries * 4);
   if (remap) {
@@ -442,7 +452,7 @@ int run_group(phobic_ctx* ctx, Job& j, hipStream_t stream) {
   for (uint32_t s = 0; s < gl.sets; s++) {
     Request& r = *j.reqs[s];
     r.block = block;
-    r.outs.pilots = bo.pilots + (uint64_t)gl.B * gl.part_base[s];
+    r.outs.pilots = bo.pilots + gl.pilot_base[s];
     r.outs.offsets = bo.offsets + gl.part_base[s] + s;
     r.outs.remap = remap ? bo.remap + gl.remap_base[s] : nullptr;
     r.outs.remap_offsets = remap ? bo.remap_offsets + gl.part_base[s] + s : nullptr;

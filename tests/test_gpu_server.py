@@ -1,6 +1,7 @@
 """Serving: requests submitted while others run give the oracle's results,
-alone or built together with other queued requests of their configuration in
-one launch; admission keeps the server's counted memory within the budget
+alone or built together in one launch with other queued requests of their
+key kind, partition size and load factor (lambda and the seed may differ
+from request to request); admission keeps the server's counted memory within the budget
 and never below what the device pool actually holds."""
 import threading
 import time
@@ -133,6 +134,48 @@ def test_large_requests_group_among_themselves():
     s = srv.stats()
     # one launch of the three large requests, one of the three small ones
     assert s["grouped_launches"] == 2 and s["grouped_requests"] == 6
+    for rid in ids + [blocker]:
+        srv.release(rid)
+    srv.close()
+
+
+@pytest.mark.parametrize("kind", ["u64", "bytes"])
+def test_mixed_lambdas_and_seeds_share_a_launch(kind):
+    """Small requests of one partition size and load factor but five lambdas
+    (four of each) and several seeds, queued behind a busy slot: launches of
+    up to four lambdas build them together (at least sixteen requests in
+    groups, whichever request the slot starts alone if the blocker ends
+    before the rest are queued), and every result equals the oracle's for
+    the request's own lambda and seed."""
+    srv = ph.Server(slots=1, hbm_budget=64 << 30)
+    blocker = srv.submit_u64(po.splitmix64(80, 60_000_000), ph.make_config(7.0, 2048))
+    rng = np.random.default_rng(5)
+    lams = [6.0, 7.0, 7.5, 8.0, 9.0]
+    reqs = []
+    for i in range(20):
+        n = int(rng.integers(5_000, 200_000))
+        lam = lams[i % 5]
+        seed = int(rng.integers(1, 1 << 63)) if i % 2 else 0
+        cfg = ph.make_config(lam, 2048, seed, 0.99)
+        if kind == "u64":
+            reqs.append((po.splitmix64(300 + i, n), None, cfg, lam, seed))
+        else:
+            m = n // 4
+            offs = np.zeros(m + 1, np.uint64)
+            offs[1:] = np.cumsum(rng.integers(8, 65, m))
+            reqs.append((rng.integers(0x20, 0x7F, int(offs[-1]), dtype=np.uint8), offs, cfg, lam, seed))
+    ids = [srv.submit_u64(a, cfg) if kind == "u64" else srv.submit_bytes(a, b, cfg) for a, b, cfg, _, _ in reqs]
+    assert srv.wait(blocker)[0] == 0
+    for rid, (a, b, _, lam, seed) in zip(ids, reqs):
+        assert srv.wait(rid)[0] == 0
+        hk = po.hash_u64(a, seed) if kind == "u64" else po.hash_bytes(a, b, seed)
+        st, pil, off, rm, roff = po.build_keys_alpha(hk, lam, 2048, 0.99)
+        assert st == 0
+        r = srv.fetch(rid)
+        assert np.array_equal(r.pilots, pil) and np.array_equal(r.offsets, off)
+        assert np.array_equal(r.remap, rm) and np.array_equal(r.remap_offsets, roff)
+    s = srv.stats()
+    assert 2 <= s["grouped_launches"] <= 3 and s["grouped_requests"] >= 16
     for rid in ids + [blocker]:
         srv.release(rid)
     srv.close()
