@@ -205,9 +205,10 @@ int phobic_batch_destroy(phobic_batch* batch);
  * Requests queue in arrival order.  Whenever one of `slots` workers is idle,
  * the scheduler takes the oldest request that fits the HBM budget (a later
  * one may overtake a request that does not fit yet) together with every
- * queued request of the same configuration (key kind, lambda, P, seed,
- * alpha) and size class: small requests (at most 4e6 keys) up to 64e6 keys
- * in all, large ones up to 256e6, 1024 requests at most.  They are built as
+ * queued request of the same key kind, P, alpha and size class, whatever
+ * its lambda (up to four lambdas per launch) and seed: small requests (at
+ * most 4e6 keys) up to 64e6 keys in all, large ones up to 256e6, 1024
+ * requests at most.  They are built as
  * one flattened launch whose per-request outputs equal building each alone.
  * Requests with a custom fulcrum table, and byte strings already in device
  * memory, run alone.  Each worker owns a context and stream, so admitted
