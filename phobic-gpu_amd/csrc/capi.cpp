@@ -281,6 +281,8 @@ struct BatchInfo {
   uint32_t ntabs = 0;
   const uint32_t* set_B = nullptr;
   const uint64_t* set_pilot_base = nullptr;
+  // u64 sets read where they lie (null: concatenated at src.u64)
+  const uint64_t* const* set_keys = nullptr;
 };
 
 // Layout of the context scratch for a plan.
@@ -417,6 +419,7 @@ int launch_build(phobic_ctx* ctx, const Scratch& sc, int kind, const KeySource& 
   ha.sets = batch.sets;
   ha.set_key_off = batch.set_key_off;
   ha.set_part_base = batch.set_part_base;
+  ha.set_keys = batch.set_keys;
   ha.set_seed = batch.set_seed;
   ha.set_ftab = batch.set_ftab;
   ha.ftabs = batch.ftabs;
@@ -710,9 +713,10 @@ uint64_t grain(uint64_t b) { return (b + kPoolGrain - 1) / kPoolGrain * kPoolGra
 struct GroupTables {
   uint64_t key_off, part_base, remap_base, goff, groff, part_remap;
   uint64_t seed, setB, ftab, pilot_base, ftabs;  // sets of several configurations
+  uint64_t keys;                                 // u64 sets read in place
   uint64_t bytes;
 };
-GroupTables group_tables(uint32_t sets, uint32_t np, uint32_t ntabs = 0) {
+GroupTables group_tables(uint32_t sets, uint32_t np, uint32_t ntabs, bool in_place) {
   GroupTables t;
   uint64_t at = 0;
   auto take = [&at](uint64_t b) {
@@ -732,6 +736,7 @@ GroupTables group_tables(uint32_t sets, uint32_t np, uint32_t ntabs = 0) {
   t.ftab = take(mixed ? 4ull * sets : 0);
   t.pilot_base = take(mixed ? 8ull * (sets + 1) : 0);
   t.ftabs = take(4ull * kFulcsTable * ntabs);
+  t.keys = take(in_place ? 8ull * sets : 0);
   t.bytes = at;
   return t;
 }
@@ -814,6 +819,7 @@ int phobic::group_layout(const phobic_config* const* cfgs, const uint64_t* set_s
   gl->sets = sets;
   gl->mixed = mixed;
   gl->ntabs = 0;
+  gl->set_keys.clear();
   uint32_t Bmax = 0;
   std::vector<double> lambdas;  // of the fulcrum tables, in table order
   uint64_t parts = 0, rem = 0, pil = 0;
@@ -861,7 +867,7 @@ uint64_t phobic::group_scratch_growth(const phobic_ctx* ctx, const phobic_config
                                       bool from_empty) {
   Plan pl;
   if (group_plan(cfg, gl, &pl) != PHOBIC_OK) return 0;
-  return growth(ctx, pl, group_tables(gl.sets, gl.np, gl.ntabs).bytes, from_empty);
+  return growth(ctx, pl, group_tables(gl.sets, gl.np, gl.ntabs, !gl.set_keys.empty()).bytes, from_empty);
 }
 
 uint64_t phobic::build_scratch_growth(const phobic_ctx* ctx, const phobic_config* cfg, uint64_t n, bool from_empty) {
@@ -957,13 +963,20 @@ int phobic::group_build(phobic_ctx* ctx, const phobic_config* cfg, int kind, con
                         const uint64_t* d_offs, const GroupLayout& gl, const phobic_outputs& block, uint32_t* d_status,
                         hipStream_t s) {
   if (!ctx || !d_status || gl.sets == 0 || (kind != kKindU64 && kind != kKindBytes)) return PHOBIC_E_INVALID_CONFIG;
-  if (gl.n && (!d_keys || (kind == kKindBytes && !d_offs))) return PHOBIC_E_INVALID_CONFIG;
+  const bool in_place = !gl.set_keys.empty();
+  if (in_place) {  // u64 sets read where they lie
+    if (kind != kKindU64 || gl.set_keys.size() != gl.sets) return PHOBIC_E_INVALID_CONFIG;
+    for (uint32_t x = 0; x < gl.sets; x++)
+      if (gl.key_off[x + 1] > gl.key_off[x] && !gl.set_keys[x]) return PHOBIC_E_INVALID_CONFIG;
+  } else if (gl.n && (!d_keys || (kind == kKindBytes && !d_offs))) {
+    return PHOBIC_E_INVALID_CONFIG;
+  }
   Plan pl;
   TRY(group_plan(cfg, gl, &pl));
   const bool remap = pl.a_ppm < 1000000u;
   if (!block.pilots || !block.offsets || (remap && (!block.remap || !block.remap_offsets))) return PHOBIC_E_INVALID_CONFIG;
   TRY(set_device(ctx));
-  const GroupTables t = group_tables(gl.sets, gl.np, gl.ntabs);
+  const GroupTables t = group_tables(gl.sets, gl.np, gl.ntabs, !gl.set_keys.empty());
   TRY_HIP(ctx->grp.ensure(t.bytes, s));
   char* g = ctx->grp.as<char>();
   // from pageable memory: each copy has read its source when it returns
@@ -982,6 +995,8 @@ int phobic::group_build(phobic_ctx* ctx, const phobic_config* cfg, int kind, con
     TRY_HIP(hipMemcpyAsync(g + t.pilot_base, gl.pilot_base.data(), 8ull * (gl.sets + 1), hipMemcpyHostToDevice, s));
     TRY_HIP(hipMemcpyAsync(g + t.ftabs, gl.ftabs.data(), 4ull * gl.ftabs.size(), hipMemcpyHostToDevice, s));
   }
+  if (in_place)
+    TRY_HIP(hipMemcpyAsync(g + t.keys, gl.set_keys.data(), 8ull * gl.sets, hipMemcpyHostToDevice, s));
   TRY(prepare_build(ctx, cfg, pl, s));
   BatchInfo bi;
   bi.sets = gl.sets;
@@ -999,6 +1014,7 @@ int phobic::group_build(phobic_ctx* ctx, const phobic_config* cfg, int kind, con
     bi.ftabs = reinterpret_cast<const uint32_t*>(g + t.ftabs);
     bi.ntabs = gl.ntabs;
   }
+  if (in_place) bi.set_keys = reinterpret_cast<const uint64_t* const*>(g + t.keys);
   KeySource src{};
   if (kind == kKindU64) {
     src.u64 = static_cast<const uint64_t*>(d_keys);

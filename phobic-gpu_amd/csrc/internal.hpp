@@ -41,6 +41,9 @@ struct GroupLayout {
   std::vector<uint64_t> seed;
   std::vector<uint32_t> setB, ftab, ftabs;
   uint32_t ntabs = 0;
+  // u64 sets read where they lie: per set the device address of its keys
+  // (empty: the sets are concatenated in the group's key array)
+  std::vector<const uint64_t*> set_keys;
 };
 // cfgs[s]: the configuration of set s
 int group_layout(const phobic_config* const* cfgs, const uint64_t* set_sizes, uint32_t sets, GroupLayout* gl);

@@ -62,6 +62,9 @@ struct HashRouteArgs {
   const uint32_t* set_ftab;
   const uint32_t* ftabs;
   uint32_t ntabs;
+  // u64 sets read in place (null: the keys are src.u64, concatenated):
+  // key i of set s is set_keys[s][i - set_key_off[s]]
+  const uint64_t* const* set_keys;
 };
 
 // Most fulcrum tables one launch holds (sets that differ in lambda).

@@ -286,7 +286,35 @@ __global__ __launch_bounds__(kT) void k_hash_route(HashRouteArgs a) {
   auto load_u64 = [&](uint64_t t, uint64_t (&dst)[kPer]) {
     const uint64_t i0 = t * kTile;
     const uint32_t cnt = (uint32_t)min((uint64_t)kTile, a.n - i0);
-    const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.src.u64 + i0, cnt * 8);
+    const uint64_t* src = nullptr;
+    if (!(SETS && a.set_keys)) {
+      src = a.src.u64 + i0;
+    } else {
+      // sets read in place: the set of the tile's first key (the last s
+      // with set_key_off[s] <= i0, wave-uniform)
+      uint32_t lo_s = 0, hi_s = a.sets;
+      while (hi_s - lo_s > 1) {
+        const uint32_t mid = (lo_s + hi_s) / 2;
+        if (a.set_key_off[mid] <= i0) lo_s = mid;
+        else hi_s = mid;
+      }
+      const uint64_t s_beg = a.set_key_off[lo_s], s_end = a.set_key_off[lo_s + 1];
+      if (i0 + cnt > s_end) {  // the tile spans sets: each key finds its own
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+          const uint64_t i = i0 + k * kT + threadIdx.x;
+          dst[k] = 0;
+          if (i < a.n) {
+            uint32_t sk = lo_s;
+            while (a.set_key_off[sk + 1] <= i) sk++;
+            dst[k] = a.set_keys[sk][i - a.set_key_off[sk]];
+          }
+        }
+        return;
+      }
+      src = a.set_keys[lo_s] + (i0 - s_beg);
+    }
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(src, cnt * 8);
 #pragma unroll
     for (uint32_t k = 0; k < kPer; k++) dst[k] = buf_load_u64(r, threadIdx.x * 8, k * kT * 8);
   };
@@ -1135,6 +1163,7 @@ hipError_t launch_hash_route(int kind, int out_fmt, const HashRouteArgs& a, uint
   // batched builds (sets) take u64 keys or byte strings; sets of several
   // configurations bring a seed and a fulcrum table index per set
   if (a.sets && kind == kKindKey) return hipErrorInvalidValue;
+  if (a.set_keys && (!a.sets || kind != kKindU64)) return hipErrorInvalidValue;
   if (a.ntabs > kMaxFulcTabs || (a.set_seed != nullptr) != (a.ntabs != 0) ||
       (a.ntabs && (!a.sets || !a.set_ftab || !a.ftabs)))
     return hipErrorInvalidValue;

@@ -217,7 +217,8 @@ std::shared_ptr<Job> form_job(phobic_server* s, std::deque<std::shared_ptr<Reque
       for (const auto& r : j->reqs) {
         sizes.push_back(r->req.n);
         cfgs.push_back(&r->cfg);
-        in += r->req.kind == PHOBIC_KEYS_BYTES ? r->req.bytes_len : r->req.n * 8;
+        // strings are staged; u64 keys in device memory are read in place
+        in += r->req.kind == PHOBIC_KEYS_BYTES ? r->req.bytes_len : r->req.keys_on_device ? 0 : r->req.n * 8;
       }
       if (phobic::group_layout(cfgs.data(), sizes.data(), (uint32_t)sizes.size(), &j->gl) != PHOBIC_OK) {
         j->reqs.resize(1);  // build it alone
@@ -373,20 +374,18 @@ int run_group(phobic_ctx* ctx, Job& j, hipStream_t stream) {
   uint64_t* d_offs = reinterpret_cast<uint64_t*>(q + align256(key_bytes));
   // Each request's keys at `at[s]` of the staging area.  Runs of requests in
   // host memory are gathered into the pinned chunks and uploaded together;
-  // device-resident u64 keys are copied on the device.
+  // u64 keys already in device memory are read where they lie (no staging;
+  // groupable strings are all in host memory).
   std::vector<uint64_t> at(gl.sets + 1, 0);
   std::vector<const char*> src(gl.sets);
   for (uint32_t k = 0; k < gl.sets; k++) {
     const Request& r = *j.reqs[k];
-    const uint64_t kb = bytes ? r.req.offsets[r.req.n] - r.req.offsets[0] : r.req.n * 8;
+    const uint64_t kb = bytes ? r.req.offsets[r.req.n] - r.req.offsets[0] : r.req.keys_on_device ? 0 : r.req.n * 8;
     src[k] = static_cast<const char*>(r.req.keys) + (bytes ? r.req.offsets[0] : 0);
     at[k + 1] = at[k] + kb;
   }
   for (uint32_t k = 0; k < gl.sets;) {
     if (j.reqs[k]->req.keys_on_device) {
-      const uint64_t kb = at[k + 1] - at[k];
-      if (kb && hipMemcpyAsync(q + at[k], src[k], kb, hipMemcpyDeviceToDevice, stream) != hipSuccess)
-        return fail(PHOBIC_E_HIP);
       k++;
       continue;
     }
@@ -441,6 +440,12 @@ int run_group(phobic_ctx* ctx, Job& j, hipStream_t stream) {
     p += align256(entries * 4);
   }
   uint32_t* d_status = reinterpret_cast<uint32_t*>(p);
+  if (!bytes) {
+    j.gl.set_keys.resize(gl.sets);
+    for (uint32_t k = 0; k < gl.sets; k++)
+      j.gl.set_keys[k] = j.reqs[k]->req.keys_on_device ? static_cast<const uint64_t*>(j.reqs[k]->req.keys)
+                                                       : reinterpret_cast<const uint64_t*>(q + at[k]);
+  }
   int st = phobic::group_build(ctx, &r0.cfg, bytes ? 1 : 0, q, bytes ? d_offs : nullptr, gl, bo, d_status, stream);
   if (st) return fail(st);
   uint32_t bits = 0;

@@ -181,6 +181,40 @@ def test_mixed_lambdas_and_seeds_share_a_launch(kind):
     srv.close()
 
 
+def test_device_resident_sets_read_in_place():
+    """A group of u64 requests, some in device memory (read where they lie)
+    and some in host memory (staged), with sets of 1 to 200,000 keys, so
+    that routing tiles span many sets: every result is the oracle's."""
+    rng = np.random.default_rng(21)
+    sizes = [1, 3, 2, 100, 5000, 1, 70_000, 17, 200_000, 9, 4096, 8191, 8193, 1, 150_000, 33]
+    srv = ph.Server(slots=1, hbm_budget=64 << 30)
+    blocker = srv.submit_u64(po.splitmix64(81, 60_000_000), ph.make_config(7.0, 2048))
+    reqs, bufs, ids = [], [], []
+    for i, n in enumerate(sizes):
+        lam = [6.0, 7.5, 8.0][i % 3]
+        seed = int(rng.integers(0, 1 << 63))
+        cfg = ph.make_config(lam, 2048, seed, 1.0)
+        keys = po.splitmix64(600 + i, n)
+        if i % 2:
+            d = hip_rt.DeviceBuffer.from_numpy(keys)
+            bufs.append(d)
+            ids.append(srv.submit_u64(d, cfg, n))
+        else:
+            ids.append(srv.submit_u64(keys, cfg))
+        reqs.append((keys, lam, seed))
+    assert srv.wait(blocker)[0] == 0
+    for rid, (keys, lam, seed) in zip(ids, reqs):
+        assert srv.wait(rid)[0] == 0
+        st, pil, off = po.build_keys(po.hash_u64(keys, seed), lam, 2048)
+        assert st == 0
+        r = srv.fetch(rid)
+        assert np.array_equal(r.pilots, pil) and np.array_equal(r.offsets, off)
+    assert srv.stats()["grouped_requests"] >= 12
+    for rid in ids + [blocker]:
+        srv.release(rid)
+    srv.close()
+
+
 def test_grouped_host_strings_rebased_through_staging():
     """A group of host string requests larger than the direct-copy limit
     (their bytes and offsets go through the pinned staging chunks, the
