@@ -16,8 +16,10 @@
 //      distinct and free; they are then marked taken.
 //
 // Slot state lives in an extended bitmap E with E[i] = taken[i mod tm] for
-// i < 2*tm + 128, so the candidate offsets [D, D+w) of a key at initial
-// position p are one unaligned w-bit read of E at p + D.  A trial scans all
+// i < 2*tm, so the candidate offsets [D, D+w) of a key at initial position p
+// are one unaligned w-bit read of E at p + D.  Every bit that decides an
+// offset d < tm lies below 2*tm (p < tm); reads past it only meet offsets
+// that the masks drop, so each slot is marked in two copies.  A trial scans all
 // offsets bit-parallel: lane l owns offsets [32 l, 32 l + 32) of a pass of
 // 2048, reads the 32-bit window of every key (consecutive words across the
 // lanes, so no bank conflicts) and ORs them; the lowest free bit of the
@@ -558,7 +560,6 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   } else {
     modc = ~0ull / (tm ? tm : 1) + 1;
   }
-  const uint32_t L = 2 * tm + 128;  // valid bits of E
   // largest pilot whose offsets [0, tm) all keep V below 2^31
   const uint32_t pilot_full = tm ? (uint32_t)(0x80000000ull / tm) - 1 : 0;
   const uint32_t* Elane = E + lane;
@@ -652,20 +653,18 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
           const bool mine = g == gi && kk < s;
           uint32_t q = unpack_slot(pk[0]) + d;
           if (q >= tm) q -= tm;
-          const uint32_t q1 = q + tm;  // both below L = 2 tm + 128
+          const uint32_t q1 = q + tm;  // both below 2 tm
           const uint32_t m0 = 1u << (q & 31), m1 = 1u << (q1 & 31);
           bool c = false;
           if (mine) {
             c = (atomicOr(&E[q >> 5], m0) & m0) != 0;
             atomicOr(&E[q1 >> 5], m1);
-            for (uint32_t x = q1 + tm; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
           }
           lds_sync();
           if (__ballot(c) == 0) return 1u;
           if (mine) {
             atomicAnd(&E[q >> 5], ~m0);
             atomicAnd(&E[q1 >> 5], ~m1);
-            for (uint32_t x = q1 + tm; x < L; x += tm) atomicAnd(&E[x >> 5], ~(1u << (x & 31)));
           }
           lds_sync();
           return 0u;
@@ -812,10 +811,9 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         if (!done && g == gsel && 64 * j < s && k < s) {
           uint32_t q = unpack_slot(pk[j]) + dsel;
           if (q >= tm) q -= tm;
-          const uint32_t q1 = q + tm;  // both below L = 2 tm + 128
+          const uint32_t q1 = q + tm;  // both below 2 tm
           atomicOr(&E[q >> 5], 1u << (q & 31));
           atomicOr(&E[q1 >> 5], 1u << (q1 & 31));
-          for (uint32_t x = q1 + tm; x < L; x += tm) atomicOr(&E[x >> 5], 1u << (x & 31));
         }
       }
       if (lane == 0) pil[cb] = dsel + tm * psel;
@@ -855,8 +853,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         scan_offsets<1, H>(Elane, pk, i, 1, tm, true, vfull, dsel);  // finds one: a slot is free
         uint32_t q = unpack_slot(__builtin_amdgcn_readlane(pk[0], i)) + dsel;
         if (q >= tm) q -= tm;
-        const uint32_t qc = q + lane * tm;  // the copies of slot q in E, one lane each
-        if (lane < 3 && qc < L) atomicOr(&E[qc >> 5], 1u << (qc & 31));
+        const uint32_t qc = q + lane * tm;  // the two copies of slot q in E, one lane each
+        if (lane < 2) atomicOr(&E[qc >> 5], 1u << (qc & 31));
         if (lane == 0) pil[__builtin_amdgcn_readlane(bi, i)] = dsel;
         lds_sync();
       }

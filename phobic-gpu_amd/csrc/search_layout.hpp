@@ -19,7 +19,7 @@ PHOBIC_HD inline uint32_t layout_max(uint32_t a, uint32_t b) { return a > b ? a 
 // o0 + 2048 h + [32 l, 32 l + 32) for h < subwindows(tm).
 PHOBIC_HD inline uint32_t subwindows(uint32_t tm) { return tm <= 2048 ? 1 : tm <= 4096 ? 2 : 4; }
 
-// E words: marks reach bit 2*tm + 128; a scan pass reads up to word
+// E words: marks reach bit 2*tm - 1; a scan pass reads up to word
 // (2*tm - 2) / 32 + 64 * subwindows(tm).
 PHOBIC_HD inline uint32_t e_words(uint32_t tcap) { return (2 * tcap) / 32 + 64 * subwindows(tcap) + 8; }
 
