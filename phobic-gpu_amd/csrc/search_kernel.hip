@@ -257,13 +257,11 @@ __device__ __forceinline__ void or_bucket(const char* eb, const uint32_t (&pk)[C
   }
 }
 
-// One pass over the offsets [0, 2048 H) with masks vfull (a table of at
-// most 2048 H slots): the smallest offset at which the s slots of lane
-// group gbase (C = 1) are all free, or ~0u.
+// ORs into acc[h] the windows at offsets [0, 2048 H) of the s slots of lane
+// group gbase (C = 1): one pass of a table of at most 2048 H slots.
 template <int H>
-__device__ __forceinline__ uint32_t scan_pass(const uint32_t* Elane, uint32_t pk, uint32_t pk2, uint32_t gbase,
-                                              uint32_t s, const uint32_t (&vfull)[H]) {
-  uint32_t acc[H];
+__device__ __forceinline__ void scan_or(const uint32_t* Elane, uint32_t pk, uint32_t pk2, uint32_t gbase, uint32_t s,
+                                        uint32_t (&acc)[H]) {
   const char* eb = reinterpret_cast<const char*>(Elane);
   if constexpr (H < 4) {  // two slots per lane read (gbase is a multiple of 4)
     // s >= 2: the first pair sets the accumulators; then the lane index
@@ -284,25 +282,27 @@ __device__ __forceinline__ uint32_t scan_pass(const uint32_t* Elane, uint32_t pk
     const uint32_t pka[1] = {pk};
     or_bucket<1, H>(eb, pka, gbase, s, acc);
   }
-  uint32_t fr[H], frall = 0;
+}
+
+// The smallest free offset of a pass: fr[h] = the free offsets of lane
+// bits, anyall = ballot(fr[0] | .. | fr[H-1] != 0), nonzero.
+template <int H>
+__device__ __forceinline__ uint32_t first_free(const uint32_t (&fr)[H], uint64_t anyall) {
+  if constexpr (H == 1) {
+    const uint32_t l0 = (uint32_t)__builtin_ctzll(anyall);
+    return 32 * l0 + (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(fr[0], l0));
+  } else {
+    uint32_t d = ~0u;
 #pragma unroll
-  for (int h = 0; h < H; h++) {
-    fr[h] = ~acc[h] & vfull[h];
-    frall |= fr[h];
-  }
-  // one vector compare decides the common case, no free offset at all
-  if (!__ballot(frall != 0)) return ~0u;
-  uint64_t any[H];
-#pragma unroll
-  for (int h = 0; h < H; h++) any[h] = __ballot(fr[h] != 0);
-#pragma unroll
-  for (int h = 0; h < H; h++) {
-    if (any[h]) {
-      const uint32_t l0 = (uint32_t)__builtin_ctzll(any[h]);
-      return 2048 * h + 32 * l0 + (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(fr[h], l0));
+    for (int h = H - 1; h >= 0; h--) {
+      const uint64_t any = __ballot(fr[h] != 0);
+      if (any) {
+        const uint32_t l0 = (uint32_t)__builtin_ctzll(any);
+        d = 2048 * h + 32 * l0 + (uint32_t)__builtin_ctz(__builtin_amdgcn_readlane(fr[h], l0));
+      }
     }
+    return d;
   }
-  return ~0u;
 }
 
 // Smallest offset d < dlimit at which the s slots of the bucket (held in pk
@@ -565,9 +565,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   const uint32_t* Elane = E + lane;
   uint32_t err = 0;
   // packed slot of key kv for pilot pl: (hash(l1, hash(l2, pl)) >> 1) mod tm
-  auto slot = [&](uint64_t kv, uint32_t pl) {
+  // (mc: std::integral_constant<bool, magic>, so that the pilot loops hold
+  // no branch on the remainder's kind)
+  auto slot = [&](uint64_t kv, uint32_t pl, auto mc) {
     const uint32_t h = mix32((uint32_t)kv, mix32((uint32_t)(kv >> 32), pl)) >> 1;
-    return pack_slot(magic ? mod_magic(h, (uint32_t)modc, (uint32_t)(modc >> 32), tm) : fastmod(h, modc, tm));
+    if constexpr (decltype(mc)::value) return pack_slot(mod_magic(h, (uint32_t)modc, (uint32_t)(modc >> 32), tm));
+    else return pack_slot(fastmod(h, modc, tm));
   };
 
   struct Bucket {
@@ -589,7 +592,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // (C = 1) hash G = 64 / 2^lg consecutive pilots per round: lane group g
   // of 2^lg >= 4 lanes holds the slots for the round's pilot g, its lanes past s
   // repeating the last key.
-  auto search_buckets = [&](auto chunks, auto subwin, uint32_t r_end) {
+  auto search_buckets = [&](auto chunks, auto subwin, uint32_t r_end, auto mc)__attribute__((always_inline)) {
     constexpr int C = decltype(chunks)::value;
     constexpr int H = decltype(subwin)::value;
     constexpr bool kPad = C == 1;
@@ -631,14 +634,17 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
 
       const uint32_t s = cur.s, cst = cur.st, cb = cur.b;
       // the d < 64 window test pays off for large buckets in a sparse table
-      const bool win0 = s >= kWin0MinKeys && 2 * placed < tm;
+      // s >= kWin0MinKeys && 2 placed < tm, as the sign bit of integer
+      // differences (no boolean, which would be kept as a lane mask)
+      const uint32_t win0 = ((kWin0MinKeys - 1 - s) & (2 * placed - tm)) >> 31;
       const uint32_t lg = clg;
       const uint32_t G = 64u >> lg;
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
       uint32_t pk[C];  // packed slots
       uint32_t pk2 = 0;  // pk[0] paired (pair_pack), in the lean loop
       uint32_t pilot = 0, dsel = 0;
-      bool dupchk = false, done = false;  // done: placed and marked by the lean loop
+      bool dupchk = false;
+      uint32_t done = 0;  // 1: placed and marked by the lean loop (wave-uniform)
       if constexpr (C == 1) {
         // The common case, with less control flow: pilots whose offsets all
         // keep V below 2^31 and that are below the duplicate check.  The
@@ -672,41 +678,72 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         // The pilot loops have one exit, at the bottom, so that the
         // compiler keeps no exit flags in them.
         if (win0) {
-          bool more = pilot < fast_end;
+          // loop state as integers (boolean state is kept as lane masks)
+          uint32_t more = pilot < fast_end ? 1u : 0u, okl = 0;
           while (more) {
             const uint32_t gi = pilot & (G - 1);
             if (gi == 0) {
-              pk[0] = slot(key[0], pilot + g);
+              pk[0] = slot(key[0], pilot + g, mc);
               pk2 = pair_pack(pk[0]);
             }
             const uint64_t t0 = g == gi ? window64(E, pk[0]) : 0;
             const uint64_t free0 = ~(((uint64_t)wave_or((uint32_t)(t0 >> 32)) << 32) | wave_or((uint32_t)t0)) &
                                    low_mask(tm);
-            const uint32_t d =
-                free0 ? (uint32_t)__builtin_ctzll(free0) : scan_pass<H>(Elane, pk[0], pk2, gi << lg, s, vfull);
             uint32_t ok = 0;  // 1: placed and marked (wave-uniform)
-            if (d != ~0u) ok = mark_distinct(gi, d);
-            dsel = d;
+            if (free0) {
+              dsel = (uint32_t)__builtin_ctzll(free0);
+              ok = mark_distinct(gi, dsel);
+            } else {
+              uint32_t acc[H], fr[H], frall = 0;
+              scan_or<H>(Elane, pk[0], pk2, gi << lg, s, acc);
+#pragma unroll
+              for (int h = 0; h < H; h++) {
+                fr[h] = ~acc[h] & vfull[h];
+                frall |= fr[h];
+              }
+              const uint64_t anyall = __ballot(frall != 0);
+              if (anyall) {
+                dsel = first_free<H>(fr, anyall);
+                ok = mark_distinct(gi, dsel);
+              }
+            }
             pilot += 1 - ok;
-            done = ok != 0;
-            more = ok == 0 && pilot < fast_end;
+            okl = ok;
+            more = pilot < fast_end ? ok ^ 1u : 0u;
+            asm volatile("" : "+s"(more));  // an integer in an SGPR, not a lane mask
           }
+          done = okl;
+          asm volatile("" : "+s"(done));
         } else {
-          bool more = pilot < fast_end;
+          // loop state as integers (boolean state is kept as lane masks)
+          uint32_t more = pilot < fast_end ? 1u : 0u, okl = 0;
           while (more) {
             const uint32_t gi = pilot & (G - 1);
             if (gi == 0) {
-              pk[0] = slot(key[0], pilot + g);
+              pk[0] = slot(key[0], pilot + g, mc);
               pk2 = pair_pack(pk[0]);
             }
-            const uint32_t d = scan_pass<H>(Elane, pk[0], pk2, gi << lg, s, vfull);
+            uint32_t acc[H], fr[H], frall = 0;
+            scan_or<H>(Elane, pk[0], pk2, gi << lg, s, acc);
+#pragma unroll
+            for (int h = 0; h < H; h++) {
+              fr[h] = ~acc[h] & vfull[h];
+              frall |= fr[h];
+            }
+            // one vector compare decides the common case, no free offset at all
+            const uint64_t anyall = __ballot(frall != 0);
             uint32_t ok = 0;  // 1: placed and marked (wave-uniform)
-            if (d != ~0u) ok = mark_distinct(gi, d);
-            dsel = d;
+            if (anyall) {
+              dsel = first_free<H>(fr, anyall);
+              ok = mark_distinct(gi, dsel);
+            }
             pilot += 1 - ok;
-            done = ok != 0;
-            more = ok == 0 && pilot < fast_end;
+            okl = ok;
+            more = pilot < fast_end ? ok ^ 1u : 0u;
+            asm volatile("" : "+s"(more));  // an integer in an SGPR, not a lane mask
           }
+          done = okl;
+          asm volatile("" : "+s"(done));
         }
       }
       for (; !done; pilot++) {
@@ -734,7 +771,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
               bool cj = false;
               pk[j] = 0;
               if (k < s) {
-                pk[j] = slot(key[j], pilot);
+                pk[j] = slot(key[j], pilot, mc);
                 const uint32_t msk = 1u << (pk[j] & 31);
                 cj = (atomicOr(&coll[pk[j] >> 7], msk) & msk) != 0;
               }
@@ -748,7 +785,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
             if ((uint32_t)j < jn && lane + 64 * j < s) coll[pk[j] >> 7] = 0;
           lds_sync();
         } else if (gi == 0) {  // slots for pilots pilot .. pilot + G - 1
-          pk[0] = slot(key[0], pilot + g);
+          pk[0] = slot(key[0], pilot + g, mc);
         }
         if (!coincide) {
           bool found = false;
@@ -805,18 +842,21 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       if (err) return;
       const uint32_t gsel = pilot & (G - 1), psel = pilot;
       // mark the bucket's slots in every copy of E (the lean loop has)
+      if (!done) {
 #pragma unroll
-      for (int j = 0; j < C; j++) {
-        const uint32_t k = kk + 64 * j;
-        if (!done && g == gsel && 64 * j < s && k < s) {
-          uint32_t q = unpack_slot(pk[j]) + dsel;
-          if (q >= tm) q -= tm;
-          const uint32_t q1 = q + tm;  // both below 2 tm
-          atomicOr(&E[q >> 5], 1u << (q & 31));
-          atomicOr(&E[q1 >> 5], 1u << (q1 & 31));
+        for (int j = 0; j < C; j++) {
+          const uint32_t k = kk + 64 * j;
+          if (g == gsel && 64 * j < s && k < s) {
+            uint32_t q = unpack_slot(pk[j]) + dsel;
+            if (q >= tm) q -= tm;
+            const uint32_t q1 = q + tm;  // both below 2 tm
+            atomicOr(&E[q >> 5], 1u << (q & 31));
+            atomicOr(&E[q1 >> 5], 1u << (q1 & 31));
+          }
         }
       }
-      if (lane == 0) pil[cb] = dsel + tm * psel;
+      // every lane stores the same value (no exec-mask switch)
+      pil[cb] = dsel + tm * psel;
       placed += s;
       lds_sync();
       cur = nxt;
@@ -835,7 +875,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // exists while keys are left, and slots of one key cannot coincide), so
   // only the first free offset from the key's slot is searched: the slots of
   // 64 of them are hashed at once, then each is scanned and marked in turn.
-  auto single_keys = [&](auto subwin) {
+  auto single_keys = [&](auto subwin, auto mc)__attribute__((always_inline)) {
     constexpr int H = decltype(subwin)::value;
     uint32_t vfull[H];
 #pragma unroll
@@ -846,7 +886,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       uint32_t pk[1] = {0}, bi = 0;
       if (lane < cnt) {
         bi = rt[r + lane] & 0xFFFFu;
-        pk[0] = slot(sorted[st0 + (r - r0) + lane], 0);
+        pk[0] = slot(sorted[st0 + (r - r0) + lane], 0, mc);
       }
       for (uint32_t i = 0; i < cnt; i++) {
         uint32_t dsel = 0;
@@ -861,19 +901,22 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     }
   };
   const uint32_t r_one = m - n_one;
-  search_buckets(I4{}, I1{}, min(r_big, m));
-  if (!err) {
+  auto search_all = [&](auto mc)__attribute__((always_inline)) {
+    search_buckets(I4{}, I1{}, min(r_big, m), mc);
+    if (err) return;
     if (tm <= 2048 || (a.mode & kSearchNarrow)) {
-      search_buckets(I1{}, I1{}, r_one);
-      if (!err) single_keys(I1{});
+      search_buckets(I1{}, I1{}, r_one, mc);
+      if (!err) single_keys(I1{}, mc);
     } else if (tm <= 4096) {
-      search_buckets(I1{}, I2{}, r_one);
-      if (!err) single_keys(I2{});
+      search_buckets(I1{}, I2{}, r_one, mc);
+      if (!err) single_keys(I2{}, mc);
     } else {
-      search_buckets(I1{}, I4{}, r_one);
-      if (!err) single_keys(I4{});
+      search_buckets(I1{}, I4{}, r_one, mc);
+      if (!err) single_keys(I4{}, mc);
     }
-  }
+  };
+  if (magic) search_all(std::true_type{});
+  else search_all(std::false_type{});
   if (err && lane == 0) atomicOr(a.status, 1u << err);
   lds_sync();
 
