@@ -267,7 +267,10 @@ __device__ __forceinline__ void scan_or(const uint32_t* Elane, uint32_t pk, uint
     // s >= 2: the first pair sets the accumulators; then the lane index
     // itself is the loop variable
     or_pairs<H, 1, true>(eb, pk2, gbase, acc);
-    const uint32_t rest = s - 2;
+    uint32_t rest = s - 2;
+    // tested here in each scan (the compiler would keep hoisted tests as
+    // lane masks)
+    asm volatile("" : "+s"(rest));
     uint32_t l = gbase + 2;
     const uint32_t lend = l + (rest & ~3u);
     for (; l < lend; l += 4) or_pairs<H, 2>(eb, pk2, l, acc);
@@ -496,8 +499,15 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     return multi && sz <= 32 ? (sz <= 4 ? 2u : 32u - __builtin_clz(sz - 1)) : 6u;
   };
 
+  // the table size of step 5 (tm = ps unless alpha < 1)
+  const uint32_t tm = __builtin_amdgcn_readfirstlane(table_size(ps, a.a_ppm));
+
   // 3. bucket start offsets: the rank table (bucket | size << 16 | lg << 25)
-  // and each bucket's cursor
+  // and each bucket's cursor.  Also r_w0: the buckets that try offsets 0..63
+  // first (at least kWin0MinKeys keys, placed while the table is at most half
+  // full) are the ranks below r_w0, since sizes fall and starts rise with
+  // the rank.
+  uint32_t r_w0 = 0;
   {
     uint32_t carry = 0;
     for (uint32_t r0 = 0; r0 < m; r0 += 64) {
@@ -505,11 +515,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t b = r < m ? rt[r] : 0;
       const uint32_t s = r < m ? bcnt[b] : 0;
       const uint32_t incl = wave_incl_scan(s);
+      const uint32_t st = carry + incl - s;
       if (r < m) {
-        const uint32_t st = carry + incl - s;
         rt[r] = b | s << 16 | lg_of(s) << 25;
         bpos[b] = st;
       }
+      r_w0 += __popcll(__ballot(r < m && s >= kWin0MinKeys && 2 * st < tm));
       carry += __builtin_amdgcn_readlane(incl, 63);
     }
   }
@@ -544,8 +555,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   for (uint32_t i = lane; i < B; i += 64) pil[i] = 0;
   lds_sync();
 
-  // 5. pilot search over a table of tm slots (tm = ps unless alpha < 1)
-  const uint32_t tm = table_size(ps, a.a_ppm);
+  // 5. pilot search over a table of tm slots
   const bool magic = tm >= kMagicModMin;
   // One 64-bit constant serves either remainder: fastmod's M for small
   // tables; for the others, mod_magic's m = ceil(2^(31+l) / tm) (low word)
@@ -561,7 +571,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     modc = ~0ull / (tm ? tm : 1) + 1;
   }
   // largest pilot whose offsets [0, tm) all keep V below 2^31
-  const uint32_t pilot_full = tm ? (uint32_t)(0x80000000ull / tm) - 1 : 0;
+  const uint32_t pilot_full = __builtin_amdgcn_readfirstlane(tm ? (uint32_t)(0x80000000ull / tm) - 1 : 0);
   const uint32_t* Elane = E + lane;
   uint32_t err = 0;
   // packed slot of key kv for pilot pl: (hash(l1, hash(l2, pl)) >> 1) mod tm
@@ -592,7 +602,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // (C = 1) hash G = 64 / 2^lg consecutive pilots per round: lane group g
   // of 2^lg >= 4 lanes holds the slots for the round's pilot g, its lanes past s
   // repeating the last key.
-  auto search_buckets = [&](auto chunks, auto subwin, uint32_t r_end, auto mc)__attribute__((always_inline)) {
+  // w0: std::integral_constant<int, W>; W = 1: every bucket of the range
+  // tries offsets 0..63 first, W = 0: none does, W = 2: decided per bucket
+  auto search_buckets = [&](auto chunks, auto subwin, uint32_t r_end, auto mc,
+                            auto w0)__attribute__((always_inline)) {
+    constexpr int W = decltype(w0)::value;
     constexpr int C = decltype(chunks)::value;
     constexpr int H = decltype(subwin)::value;
     constexpr bool kPad = C == 1;
@@ -614,6 +628,10 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     uint32_t vfull[H];  // first-pass offset masks for dlimit = tm
 #pragma unroll
     for (int h = 0; h < H; h++) vfull[h] = below_mask(2048 * h + 32 * lane, tm);
+    // the end of the lean loop's pilots (one pass must cover the table: not
+    // in the narrow mode above 2048 slots)
+    const uint32_t fast_end =
+        __builtin_amdgcn_readfirstlane(tm <= 2048 * H ? min(kDupCheckPilot, pilot_full + 1) : 0u);
     Bucket cur = bucket_at(r, placed);
     uint32_t clg = C == 1 ? cur.lg : 6u;
     uint64_t key[C];  // the current bucket's keys, in the lane-group layout
@@ -623,20 +641,18 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0) (gfx9 encoding)
     for (; r < r_end; r++) {
       // prefetch the next bucket's keys
-      Bucket nxt;
-      uint32_t nlg = 6;
+      // (the last bucket reads its own entry again: no defaults to set)
+      const Bucket nxt = bucket_at(min(r + 1, r_end - 1), placed + cur.s);
+      const uint32_t nlg = C == 1 ? nxt.lg : 6u;
       uint64_t nkey[C];
-      if (r + 1 < r_end) {
-        nxt = bucket_at(r + 1, placed + cur.s);
-        nlg = C == 1 ? nxt.lg : 6u;
-        load_keys(nxt, nlg, nkey);
-      }
+      if (r + 1 < r_end) load_keys(nxt, nlg, nkey);
 
       const uint32_t s = cur.s, cst = cur.st, cb = cur.b;
       // the d < 64 window test pays off for large buckets in a sparse table
       // s >= kWin0MinKeys && 2 placed < tm, as the sign bit of integer
       // differences (no boolean, which would be kept as a lane mask)
-      const uint32_t win0 = ((kWin0MinKeys - 1 - s) & (2 * placed - tm)) >> 31;
+      uint32_t win0 = W;
+      if constexpr (W == 2) win0 = ((kWin0MinKeys - 1 - s) & (2 * placed - tm)) >> 31;
       const uint32_t lg = clg;
       const uint32_t G = 64u >> lg;
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
@@ -649,8 +665,6 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         // The common case, with less control flow: pilots whose offsets all
         // keep V below 2^31 and that are below the duplicate check.  The
         // bucket has at least two keys (one-key buckets are placed apart).
-        // (one pass must cover the table: not in the narrow mode above 2048)
-        const uint32_t fast_end = tm <= 2048 * H ? min(kDupCheckPilot, pilot_full + 1) : 0;
         // Marks pilot group gi's slots at offset d in every copy of E and
         // tells whether they are pairwise distinct.  The scan found them all
         // free, so a bit that is already set means two of the bucket's own
@@ -902,16 +916,23 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   };
   const uint32_t r_one = m - n_one;
   auto search_all = [&](auto mc)__attribute__((always_inline)) {
-    search_buckets(I4{}, I1{}, min(r_big, m), mc);
+    using W0 = std::integral_constant<int, 0>;
+    using W1 = std::integral_constant<int, 1>;
+    using W2 = std::integral_constant<int, 2>;
+    search_buckets(I4{}, I1{}, min(r_big, m), mc, W2{});
     if (err) return;
+    const uint32_t r_w = __builtin_amdgcn_readfirstlane(min(r_w0, r_one));
     if (tm <= 2048 || (a.mode & kSearchNarrow)) {
-      search_buckets(I1{}, I1{}, r_one, mc);
+      search_buckets(I1{}, I1{}, r_w, mc, W1{});
+      if (!err) search_buckets(I1{}, I1{}, r_one, mc, W0{});
       if (!err) single_keys(I1{}, mc);
     } else if (tm <= 4096) {
-      search_buckets(I1{}, I2{}, r_one, mc);
+      search_buckets(I1{}, I2{}, r_w, mc, W1{});
+      if (!err) search_buckets(I1{}, I2{}, r_one, mc, W0{});
       if (!err) single_keys(I2{}, mc);
     } else {
-      search_buckets(I1{}, I4{}, r_one, mc);
+      search_buckets(I1{}, I4{}, r_w, mc, W1{});
+      if (!err) search_buckets(I1{}, I4{}, r_one, mc, W0{});
       if (!err) single_keys(I4{}, mc);
     }
   };
