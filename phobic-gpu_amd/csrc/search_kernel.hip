@@ -178,7 +178,8 @@ __device__ __forceinline__ void or_windows(const char* eb, uint32_t pk, uint32_t
 // (DPP quad_perm [1,0,3,2]); packed slots of tables up to 4096 slots are
 // below 2^14.
 __device__ __forceinline__ uint32_t pair_pack(uint32_t pk) {
-  return pk | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)pk, 0xB1, 0xF, 0xF, false) << 16;
+  // (shifted before the lane move, so that the move folds into the OR)
+  return pk | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(pk << 16), 0xB1, 0xF, 0xF, false);
 }
 
 // ORs into acc[h] the windows of the 2 NP slots held by lanes lane0 ..
@@ -577,11 +578,12 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // packed slot of key kv for pilot pl: (hash(l1, hash(l2, pl)) >> 1) mod tm
   // (mc: std::integral_constant<bool, magic>, so that the pilot loops hold
   // no branch on the remainder's kind)
-  auto slot = [&](uint64_t kv, uint32_t pl, auto mc) {
+  auto slot_q = [&](uint64_t kv, uint32_t pl, auto mc) {  // unpacked
     const uint32_t h = mix32((uint32_t)kv, mix32((uint32_t)(kv >> 32), pl)) >> 1;
-    if constexpr (decltype(mc)::value) return pack_slot(mod_magic(h, (uint32_t)modc, (uint32_t)(modc >> 32), tm));
-    else return pack_slot(fastmod(h, modc, tm));
+    if constexpr (decltype(mc)::value) return mod_magic(h, (uint32_t)modc, (uint32_t)(modc >> 32), tm);
+    else return fastmod(h, modc, tm);
   };
+  auto slot = [&](uint64_t kv, uint32_t pl, auto mc) { return pack_slot(slot_q(kv, pl, mc)); };
 
   struct Bucket {
     uint32_t b = 0, s = 0, lg = 6, st = 0;  // index, keys, lane-group log2, start in the slice (wave-uniform)
@@ -658,6 +660,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
       uint32_t pk[C];  // packed slots
       uint32_t pk2 = 0;  // pk[0] paired (pair_pack), in the lean loop
+      uint32_t qs = 0;   // pk[0] unpacked, in the lean loop
       uint32_t pilot = 0, dsel = 0;
       bool dupchk = false;
       uint32_t done = 0;  // 1: placed and marked by the lean loop (wave-uniform)
@@ -671,7 +674,7 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
         // slots coincide: the marks are then taken back.
         auto mark_distinct = [&](uint32_t gi, uint32_t d) {
           const bool mine = g == gi && kk < s;
-          uint32_t q = unpack_slot(pk[0]) + d;
+          uint32_t q = qs + d;
           if (q >= tm) q -= tm;
           const uint32_t q1 = q + tm;  // both below 2 tm
           const uint32_t m0 = 1u << (q & 31), m1 = 1u << (q1 & 31);
@@ -697,7 +700,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
           while (more) {
             const uint32_t gi = pilot & (G - 1);
             if (gi == 0) {
-              pk[0] = slot(key[0], pilot + g, mc);
+              qs = slot_q(key[0], pilot + g, mc);
+              pk[0] = pack_slot(qs);
               pk2 = pair_pack(pk[0]);
             }
             const uint64_t t0 = g == gi ? window64(E, pk[0]) : 0;
@@ -734,7 +738,8 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
           while (more) {
             const uint32_t gi = pilot & (G - 1);
             if (gi == 0) {
-              pk[0] = slot(key[0], pilot + g, mc);
+              qs = slot_q(key[0], pilot + g, mc);
+              pk[0] = pack_slot(qs);
               pk2 = pair_pack(pk[0]);
             }
             uint32_t acc[H], fr[H], frall = 0;
