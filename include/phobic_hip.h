@@ -274,8 +274,10 @@ int phobic_server_destroy(phobic_server* server);
  * search_mode: pilot-search variants, a mask of PHOBIC_SEARCH_* (0 = all
  *   of the fast paths);
  * split_per: records per thread of a tile of levels 2..L (0 = route_per);
- * first_fanout: at most this many level-1 bins (0 = max_fanout); the later
- *   levels then take the larger share of the fan-out;
+ * first_fanout: at most this many level-1 bins (0 = max_fanout, except
+ *   that a two-level plan whose balanced fan-out would exceed 576 keeps
+ *   level 1 at 576 bins); the later levels then take the larger share of
+ *   the fan-out;
  * route_threads: threads of a routing workgroup, 512 (0) or 1024; with 512
  *   route_per and split_per are 8, 12, 16, 20 or 24, with 1024 they are 4,
  *   6, 8, 10 or 12.  A level whose tile would not fit the workgroup's LDS
@@ -283,7 +285,7 @@ int phobic_server_destroy(phobic_server* server);
  *   and splits of at most 1024 parent bins, run persistent workgroups (one
  *   per CU the device holds) over tiles of this shape.
  * Defaults {1, 12, 1024, 0, 12, 0, 1024}: tiles of 12,288 records, and two
- * levels of about sqrt(np) bins for a billion keys.
+ * levels of 576 x 848 bins for a billion keys (P 2048).
  * phobic_autotune times eight tile shapes x first_fanout {0, 48, 512} x
  * max_fanout {128, 1024} (on the partitioning stages' device time) and then
  * the search shapes {1,2,4,8} x {0,1,2,3} (on the search stage's time) on

@@ -579,7 +579,11 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
   // (mc: std::integral_constant<bool, magic>, so that the pilot loops hold
   // no branch on the remainder's kind)
   auto slot_q = [&](uint64_t kv, uint32_t pl, auto mc) {  // unpacked
-    const uint32_t h = mix32((uint32_t)kv, mix32((uint32_t)(kv >> 32), pl)) >> 1;
+    uint32_t h = mix32((uint32_t)kv, mix32((uint32_t)(kv >> 32), pl));
+    // (k >> 16) ^ k is one instruction; folded into the shift below it
+    // would become two shifts and an xor
+    asm volatile("" : "+v"(h));
+    h >>= 1;
     if constexpr (decltype(mc)::value) return mod_magic(h, (uint32_t)modc, (uint32_t)(modc >> 32), tm);
     else return fastmod(h, modc, tm);
   };
@@ -659,8 +663,9 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t G = 64u >> lg;
       const uint32_t g = lane >> lg, kk = lane & ((1u << lg) - 1);
       uint32_t pk[C];  // packed slots
-      uint32_t pk2 = 0;  // pk[0] paired (pair_pack), in the lean loop
-      uint32_t qs = 0;   // pk[0] unpacked, in the lean loop
+      // pk[0] paired (pair_pack) and unpacked, in the lean loop (set by its
+      // first hash round, pilot 0)
+      uint32_t pk2, qs;
       uint32_t pilot = 0, dsel = 0;
       bool dupchk = false;
       uint32_t done = 0;  // 1: placed and marked by the lean loop (wave-uniform)
@@ -678,16 +683,20 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
           if (q >= tm) q -= tm;
           const uint32_t q1 = q + tm;  // both below 2 tm
           const uint32_t m0 = 1u << (q & 31), m1 = 1u << (q1 & 31);
-          bool c = false;
+          // word indices kept as such, so that each address is one
+          // shift-and-add
+          uint32_t w0 = q >> 5, w1 = q1 >> 5;
+          asm volatile("" : "+v"(w0), "+v"(w1));
+          uint32_t old = 0;  // (tested after the branch: one compare gives the ballot)
           if (mine) {
-            c = (atomicOr(&E[q >> 5], m0) & m0) != 0;
-            atomicOr(&E[q1 >> 5], m1);
+            old = atomicOr(&E[w0], m0);
+            atomicOr(&E[w1], m1);
           }
           lds_sync();
-          if (__ballot(c) == 0) return 1u;
+          if (__ballot((old & m0) != 0) == 0) return 1u;
           if (mine) {
-            atomicAnd(&E[q >> 5], ~m0);
-            atomicAnd(&E[q1 >> 5], ~m1);
+            atomicAnd(&E[w0], ~m0);
+            atomicAnd(&E[w1], ~m1);
           }
           lds_sync();
           return 0u;
