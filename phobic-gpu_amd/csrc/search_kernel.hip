@@ -478,17 +478,19 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) __attribute__((amdgpu_waves_pe
       const uint32_t v = __builtin_amdgcn_readlane(s, leader);
       const bool mine = act && s == v;
       const uint64_t same = __ballot(mine);
+      // Each size of the chunk has its own bin t, so no iteration reads what
+      // another of this chunk writes: the chunk's writes are waited for
+      // once, after it (a wave's LDS operations execute in order).
       const uint32_t t = 256 - v;
       const uint32_t bv = hist[t];
       if (mine) {
         rt[bv + __popcll(same & lanemask_lt())] = (uint32_t)idx;
         act = false;
       }
-      lds_sync();
       if (lane == leader) hist[t] = bv + __popcll(same);
-      lds_sync();
       am &= ~same;
     }
+    lds_sync();
   }
   lds_sync();
 

@@ -195,6 +195,35 @@ def test_bucket_order_is_size_desc_index_desc():
         assert sizes.sum() == off[p + 1] - off[p]
 
 
+@pytest.mark.parametrize("n,lam,P,alpha", [
+    (60000, 7.5, 2048, 0.99), (60000, 7.0, 2048, 1.0), (40000, 3.0, 500, 1.0), (50000, 12.0, 4000, 0.9),
+    (30000, 1.0, 256, 1.0),
+])
+def test_window0_buckets_lead_the_processing_order(n, lam, P, alpha):
+    """The search kernel gives the buckets that try offsets 0..63 first (at
+    least 6 keys, started while the table is at most half full) a loop of
+    their own over ranks [0, r_w0), r_w0 counted from the starts.  That is
+    exact only if those buckets are a leading run of the processing order:
+    checked here on the oracle's order for every partition."""
+    hk = _keys(n, seed=31)
+    _, _, off, bs, order = po.build_keys(hk, lam, P, intermediates=True)
+    np_, B = po.sizes(n, lam, P)
+    lib = po.load()
+    a = po.alpha_ppm(alpha)
+    leading_runs = 0
+    for p in range(np_):
+        m = int(lib.orc_table_size(int(off[p + 1] - off[p]), a))
+        nb = int((bs[p * B:(p + 1) * B] > 0).sum())
+        sizes = bs[p * B + order[p * B:p * B + nb].astype(np.int64)].astype(np.int64)
+        starts = np.concatenate(([0], np.cumsum(sizes)[:-1]))
+        w0 = (sizes >= 6) & (2 * starts < m)
+        k = int(w0.sum())
+        assert w0[:k].all() and not w0[k:].any(), p
+        leading_runs += k > 0
+    if lam >= 7.0:
+        assert leading_runs > 0  # the configuration exercises the window loop
+
+
 def _mix_np(key, pilot):
     key = (key ^ np.uint32(pilot)).astype(np.uint32)
     with np.errstate(over="ignore"):
