@@ -288,8 +288,9 @@ __device__ __forceinline__ void scan_or(const uint32_t* Elane, uint32_t pk, uint
   }
 }
 
-// The smallest free offset of a pass: fr[h] = the free offsets of lane
-// bits, anyall = ballot(fr[0] | .. | fr[H-1] != 0), nonzero.
+// The smallest free offset of a pass.  fr[h]: this lane's free offsets of
+// sub-window h (bit i: offset 2048 h + 32 lane + i); anyall: the ballot of
+// the lanes with any, nonzero.
 template <int H>
 __device__ __forceinline__ uint32_t first_free(const uint32_t (&fr)[H], uint64_t anyall) {
   if constexpr (H == 1) {
